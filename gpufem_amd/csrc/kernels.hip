@@ -1,0 +1,369 @@
+// Device kernels for the symbolic expansion, the numeric assembly (Ke by
+// Gauss quadrature fused with the colored scatter) and the penalty
+// constraint.  gfx950 (CDNA4): 64-wide wavefronts, 256-thread workgroups.
+#include <cfloat>
+
+#include "gf_internal.hpp"
+#include "launch.hpp"
+
+namespace gf {
+namespace {
+
+// ---------------------------------------------------------------------------
+// Element tables: dN/dxi at every Gauss point and the Gauss weights, per
+// (element type, points per direction).  Block (type-1)*2 + (ngauss-2) starts
+// at c_off[block]: ngp*dim*nen derivatives (dN[(g*dim + i)*nen + a]) followed
+// by ngp weights.
+// ---------------------------------------------------------------------------
+constexpr int kTabCap = 3400;
+__constant__ double c_tab[kTabCap];
+int h_off[8];
+int h_ngp[8];
+
+inline int tab_block(int type, int ngauss) { return (type - 1) * 2 + (ngauss - 2); }
+
+template <int DIM>
+__device__ inline double det_inv(const double (&J)[DIM][DIM], double (&Ji)[DIM][DIM]) {
+  if constexpr (DIM == 2) {
+    double d = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    Ji[0][0] = J[1][1] / d;
+    Ji[0][1] = -J[0][1] / d;
+    Ji[1][0] = -J[1][0] / d;
+    Ji[1][1] = J[0][0] / d;
+    return d;
+  } else {
+    double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    double d = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    Ji[0][0] = c00 / d;
+    Ji[1][0] = c01 / d;
+    Ji[2][0] = c02 / d;
+    Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / d;
+    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / d;
+    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / d;
+    Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / d;
+    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / d;
+    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / d;
+    return d;
+  }
+}
+
+struct KArgs {
+  const int32_t* conn;
+  const double* coords;
+  const double* elemE;
+  double E;
+  double c0, c1, c2;
+  const int32_t* elems;
+  int64_t nce;
+  const uint8_t* slot;
+  double* val;
+  int64_t n;
+  int* bad;
+  int tab;  // offset of the (type, rule) block in c_tab
+};
+
+// One workgroup handles EPB elements of one color.
+//  phase 0: element node ids and coordinates -> LDS
+//  phase 1: one thread per (element, Gauss point): J, detJ, J^-1, dN/dx -> LDS
+//  phase 2: one thread per (element, node pair a-b): the dim x dim block
+//           K_ab = sum_g G_a^T D G_b w_g detJ_g E_e, accumulated in registers
+//  phase 3: the same thread adds its block into the ELL values.  Elements of
+//           one color share no node, so the read-modify-write is race-free.
+template <int DIM, int NEN, int NGP>
+__global__ __launch_bounds__(256) void k_assemble(KArgs a) {
+  constexpr int PAIRS = NEN * NEN;
+  constexpr int EPB = PAIRS >= 256 ? 1 : 256 / PAIRS;
+  __shared__ double s_dndx[EPB][NGP][DIM][NEN];
+  __shared__ double s_sc[EPB][NGP];
+  __shared__ double s_x[EPB][DIM][NEN];
+  __shared__ int32_t s_node[EPB][NEN];
+  __shared__ int32_t s_elem[EPB];
+  const int tid = threadIdx.x;
+  const int64_t base = int64_t(blockIdx.x) * EPB;
+
+  for (int t = tid; t < EPB * NEN; t += 256) {
+    const int el = t / NEN, an = t % NEN;
+    if (base + el < a.nce) {
+      const int32_t e = a.elems[base + el];
+      if (an == 0) s_elem[el] = e;
+      const int32_t node = a.conn[int64_t(e) * NEN + an] - 1;
+      s_node[el][an] = node;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) s_x[el][d][an] = a.coords[int64_t(node) * DIM + d];
+    }
+  }
+  __syncthreads();
+
+  for (int t = tid; t < EPB * NGP; t += 256) {
+    const int el = t / NGP, g = t % NGP;
+    if (base + el >= a.nce) continue;
+    const double* dN = c_tab + a.tab + g * DIM * NEN;
+    double J[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) {
+        double acc = 0.0;
+#pragma unroll
+        for (int n = 0; n < NEN; ++n) acc += dN[i * NEN + n] * s_x[el][j][n];
+        J[i][j] = acc;
+      }
+    double Ji[DIM][DIM];
+    const double det = det_inv<DIM>(J, Ji);
+    if (!(det > 0.0)) *a.bad = 1;
+#pragma unroll
+    for (int n = 0; n < NEN; ++n)
+#pragma unroll
+      for (int i = 0; i < DIM; ++i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) acc += Ji[i][k] * dN[k * NEN + n];
+        s_dndx[el][g][i][n] = acc;
+      }
+    const double w = c_tab[a.tab + NGP * DIM * NEN + g];
+    const double Ee = a.elemE ? a.elemE[s_elem[el]] : a.E;
+    s_sc[el][g] = w * det * Ee;
+  }
+  __syncthreads();
+
+  for (int t = tid; t < EPB * PAIRS; t += 256) {
+    const int el = t / PAIRS, p = t % PAIRS;
+    if (base + el >= a.nce) continue;
+    const int an = p / NEN, bn = p % NEN;
+    double k[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) k[i][j] = 0.0;
+    for (int g = 0; g < NGP; ++g) {
+      const double sc = s_sc[el][g];
+      double ga[DIM], gb[DIM];
+#pragma unroll
+      for (int i = 0; i < DIM; ++i) {
+        ga[i] = s_dndx[el][g][i][an];
+        gb[i] = s_dndx[el][g][i][bn];
+      }
+      if constexpr (DIM == 2) {
+        // plane stress, D = [[c0, c1, 0], [c1, c0, 0], [0, 0, c2]]
+        k[0][0] += sc * (a.c0 * ga[0] * gb[0] + a.c2 * ga[1] * gb[1]);
+        k[0][1] += sc * (a.c1 * ga[0] * gb[1] + a.c2 * ga[1] * gb[0]);
+        k[1][0] += sc * (a.c1 * ga[1] * gb[0] + a.c2 * ga[0] * gb[1]);
+        k[1][1] += sc * (a.c0 * ga[1] * gb[1] + a.c2 * ga[0] * gb[0]);
+      } else {
+        // isotropic 3D: normal diagonal c0, normal off-diagonal c1, shear c2
+        const double dot = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            if (i == j)
+              k[i][i] += sc * (a.c0 * ga[i] * gb[i] + a.c2 * (dot - ga[i] * gb[i]));
+            else
+              k[i][j] += sc * (a.c1 * ga[i] * gb[j] + a.c2 * ga[j] * gb[i]);
+          }
+      }
+    }
+    const int32_t e = s_elem[el];
+    const int64_t slot = a.slot[(int64_t(e) * NEN + an) * NEN + bn];
+    const int64_t row0 = int64_t(s_node[el][an]) * DIM;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) {
+        double* v = a.val + (row0 + i) + (slot * DIM + j) * a.n;
+        *v += k[i][j];
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ELL expansion: row dim*v+c gets columns dim*w+c' for w in adj(v).
+// ---------------------------------------------------------------------------
+__global__ void k_ell_expand(int64_t n, int32_t width, int dim, const int64_t* adj_ptr,
+                             const int32_t* adj, double* val, int32_t* col,
+                             int32_t* rownnz) {
+  const int64_t total = n * int64_t(width);
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t row = t % n;
+    const int32_t s = int32_t(t / n);
+    const int64_t v = row / dim;
+    const int64_t p0 = adj_ptr[v];
+    const int32_t rn = int32_t(adj_ptr[v + 1] - p0) * dim;
+    int32_t c = int32_t(row);
+    if (s < rn) c = adj[p0 + s / dim] * dim + s % dim;
+    col[t] = c;
+    val[t] = 0.0;
+    if (s == 0) rownnz[row] = rn;
+  }
+}
+
+__global__ void k_diag_slot(int64_t n, int dim, const int64_t* adj_ptr, const int32_t* adj,
+                            int32_t* diag) {
+  for (int64_t row = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; row < n;
+       row += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t v = row / dim;
+    int64_t lo = adj_ptr[v], hi = adj_ptr[v + 1];
+    const int64_t p0 = lo;
+    while (lo < hi) {
+      int64_t mid = (lo + hi) >> 1;
+      if (adj[mid] < v) lo = mid + 1;
+      else hi = mid;
+    }
+    diag[row] = (lo < adj_ptr[v + 1] && adj[lo] == v)
+                    ? int32_t((lo - p0) * dim + row % dim)
+                    : -1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Penalty (src/FEM/fem.cc:112-145)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_diag_max(const double* val, const int32_t* diag,
+                                                  int64_t n, double* partials) {
+  __shared__ double sh[4];
+  double m = -DBL_MAX;
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * 256) {
+    const int32_t s = diag[i];
+    const double v = s >= 0 ? val[i + int64_t(s) * n] : 0.0;
+    m = fmax(m, v);
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+__global__ __launch_bounds__(256) void k_max_finish(const double* partials, int np,
+                                                    double coef, double* result) {
+  __shared__ double sh[4];
+  double m = -DBL_MAX;
+  for (int i = threadIdx.x; i < np; i += 256) m = fmax(m, partials[i]);
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) result[0] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3])) * coef;
+}
+
+__global__ void k_penalty_apply(double* val, const int32_t* diag, int64_t n,
+                                const int32_t* dofs, const int32_t* counts, int64_t nd,
+                                const double* pen) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < nd;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t d = dofs[t];
+    const int32_t s = diag[d];
+    if (s < 0) continue;
+    double* v = val + d + int64_t(s) * n;
+    double x = *v;
+    const double p = *pen;
+    for (int c = 0; c < counts[t]; ++c) x = p * x;
+    *v = x;
+  }
+}
+
+inline int grid_for(int64_t work, int threads, int cap) {
+  int64_t b = (work + threads - 1) / threads;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return int(b);
+}
+
+template <int DIM, int NEN, int NGP>
+hipError_t launch_asm(hipStream_t s, const KArgs& k) {
+  constexpr int PAIRS = NEN * NEN;
+  constexpr int EPB = PAIRS >= 256 ? 1 : 256 / PAIRS;
+  const int64_t blocks = (k.nce + EPB - 1) / EPB;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_assemble<DIM, NEN, NGP>), dim3(unsigned(blocks)), dim3(256), 0, s, k);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t upload_element_tables() {
+  static double host[kTabCap];
+  int off = 0;
+  for (int type = GF_Q4; type <= GF_BRICK20; ++type) {
+    ElemInfo ei;
+    elem_info(type, &ei);
+    for (int ng = 2; ng <= 3; ++ng) {
+      double pts[27 * 3], w[27];
+      const int ngp = gauss_rule(ei.dim, ng, pts, w);
+      const int blk = tab_block(type, ng);
+      h_off[blk] = off;
+      h_ngp[blk] = ngp;
+      for (int g = 0; g < ngp; ++g) shape_derivs(type, pts + g * ei.dim, host + off + g * ei.dim * ei.nen);
+      off += ngp * ei.dim * ei.nen;
+      for (int g = 0; g < ngp; ++g) host[off + g] = w[g];
+      off += ngp;
+    }
+  }
+  if (off > kTabCap) return hipErrorInvalidValue;
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_tab), host, sizeof(double) * size_t(off));
+}
+
+hipError_t launch_ell_expand(hipStream_t s, int64_t n, int32_t width, int dim,
+                             const int64_t* adj_ptr, const int32_t* adj, double* val,
+                             int32_t* col, int32_t* rownnz, int32_t* diag) {
+  const int64_t total = n * int64_t(width);
+  hipLaunchKernelGGL(k_ell_expand, dim3(grid_for(total, 256, 65536)), dim3(256), 0, s, n,
+                     width, dim, adj_ptr, adj, val, col, rownnz);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_diag_slot, dim3(grid_for(n, 256, 65536)), dim3(256), 0, s, n, dim,
+                     adj_ptr, adj, diag);
+  return hipGetLastError();
+}
+
+hipError_t launch_assembly_color(hipStream_t s, const AssemblyArgs& a) {
+  ElemInfo ei;
+  if (!elem_info(a.type, &ei) || (a.ngauss != 2 && a.ngauss != 3)) return hipErrorInvalidValue;
+  KArgs k;
+  k.conn = a.conn;
+  k.coords = a.coords;
+  k.elemE = a.elemE;
+  k.E = a.E;
+  k.c0 = a.c[0];
+  k.c1 = a.c[1];
+  k.c2 = a.c[2];
+  k.elems = a.elems;
+  k.nce = a.nce;
+  k.slot = a.slot;
+  k.val = a.val;
+  k.n = a.n;
+  k.bad = a.bad_jac;
+  k.tab = h_off[tab_block(a.type, a.ngauss)];
+  const bool g3 = a.ngauss == 3;
+  switch (a.type) {
+    case GF_Q4: return g3 ? launch_asm<2, 4, 9>(s, k) : launch_asm<2, 4, 4>(s, k);
+    case GF_Q8: return g3 ? launch_asm<2, 8, 9>(s, k) : launch_asm<2, 8, 4>(s, k);
+    case GF_BRICK8: return g3 ? launch_asm<3, 8, 27>(s, k) : launch_asm<3, 8, 8>(s, k);
+    case GF_BRICK20: return g3 ? launch_asm<3, 20, 27>(s, k) : launch_asm<3, 20, 8>(s, k);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_penalty_factor(hipStream_t s, const double* val, const int32_t* diag,
+                                 int64_t n, double coef, double* partials, int nparts,
+                                 double* result) {
+  const int nb = grid_for(n, 256, nparts);
+  hipLaunchKernelGGL(k_diag_max, dim3(nb), dim3(256), 0, s, val, diag, n, partials);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_max_finish, dim3(1), dim3(256), 0, s, partials, nb, coef, result);
+  return hipGetLastError();
+}
+
+hipError_t launch_penalty_apply(hipStream_t s, double* val, const int32_t* diag, int64_t n,
+                                const int32_t* dofs, const int32_t* counts, int64_t nd,
+                                const double* pen) {
+  if (nd == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_penalty_apply, dim3(grid_for(nd, 256, 4096)), dim3(256), 0, s, val,
+                     diag, n, dofs, counts, nd, pen);
+  return hipGetLastError();
+}
+
+}  // namespace gf

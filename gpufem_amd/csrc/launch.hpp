@@ -1,0 +1,89 @@
+// Host-callable launchers for the kernels in kernels.hip and solver.hip.
+// Every launcher enqueues on the given stream and returns the launch status.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gf {
+
+// ---- element tables --------------------------------------------------------
+// Uploads the per-(type, rule) dN/dxi and weight tables to the current device.
+hipError_t upload_element_tables();
+
+// ---- symbolic expansion ------------------------------------------------------
+hipError_t launch_ell_expand(hipStream_t s, int64_t n, int32_t width, int dim,
+                             const int64_t* adj_ptr, const int32_t* adj, double* val,
+                             int32_t* col, int32_t* rownnz, int32_t* diag);
+
+// ---- numeric assembly --------------------------------------------------------
+struct AssemblyArgs {
+  int type, ngauss;
+  const int32_t* conn;
+  const double* coords;
+  const double* elemE;  // may be null
+  double E;
+  double c[3];          // elastic coefficients per unit E
+  const int32_t* elems; // element ids of one color
+  int64_t nce;          // number of elements in the color
+  const uint8_t* slot;
+  double* val;
+  int64_t n;
+  int* bad_jac;         // set to 1 when some detJ <= 0
+};
+hipError_t launch_assembly_color(hipStream_t s, const AssemblyArgs& a);
+
+// ---- penalty -------------------------------------------------------------------
+// result[0] <- coef * max_i K_ii (0 for rows with no stored diagonal)
+hipError_t launch_penalty_factor(hipStream_t s, const double* val, const int32_t* diag,
+                                 int64_t n, double coef, double* partials, int nparts,
+                                 double* result);
+// K_dd <- pen^count_d * K_dd (applied as count repeated multiplications)
+hipError_t launch_penalty_apply(hipStream_t s, double* val, const int32_t* diag, int64_t n,
+                                const int32_t* dofs, const int32_t* counts, int64_t nd,
+                                const double* pen);
+
+// ---- SpMV / solver -----------------------------------------------------------------
+hipError_t launch_spmv(hipStream_t s, int64_t n, int32_t width, const double* val,
+                       const int32_t* col, const int32_t* rownnz, const double* x,
+                       double* y, int nrhs);
+hipError_t launch_jacobi_inv(hipStream_t s, int64_t n, const double* val,
+                             const int32_t* diag, double* dinv);
+
+struct PcgState;  // device-resident scalars (solver.hip)
+
+struct PcgBuffers {
+  int64_t n;
+  int32_t width;
+  const double* val;
+  const int32_t* col;
+  const int32_t* rownnz;
+  const double* dinv;  // null: no preconditioner (reference CG)
+  const double* b;
+  double* x;
+  double* r;
+  double* z;
+  double* p;
+  double* q;
+  double* part1;  // per-block partials of p.q
+  double* part2;  // per-block partials of (r.z, r.r)
+  int nblocks;    // grid of the vector kernels (<= max partials)
+  PcgState* st;
+};
+size_t pcg_state_bytes();
+hipError_t launch_pcg_setup(hipStream_t s, const PcgBuffers& B, int use_x0, double rtol,
+                            int maxit);
+hipError_t launch_pcg_iteration(hipStream_t s, const PcgBuffers& B, int parity);
+// Host view of the scalar state.
+struct PcgHostState {
+  int32_t it, done, status, pad;
+  double rr, bnorm2, rho;
+};
+hipError_t read_pcg_state(hipStream_t s, const PcgBuffers& B, PcgHostState* h);
+// nrep launches of the iteration's SpMV kernel with the state forced live
+hipError_t launch_spmv_pq_repeat(hipStream_t s, const PcgBuffers& B, int nrep);
+// true residual ||b - K x|| / ||b|| into *out (device), deterministic
+hipError_t launch_true_residual(hipStream_t s, const PcgBuffers& B, double* out_dev);
+
+}  // namespace gf

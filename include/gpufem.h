@@ -1,0 +1,225 @@
+/*
+ * gpufem.h -- C-ABI of the MI355X (gfx950) assembly-and-solve engine.
+ *
+ * Plain C types only: handles are opaque, arrays are caller-owned host
+ * pointers (copied in or out) unless a function says it takes device
+ * pointers.  Every function returns GF_OK (0) or a negative GF_ERR_* code;
+ * gf_last_error() gives the message.  One gf_ctx owns one HIP stream and is
+ * not thread-safe; use one context per host thread.  No function changes
+ * global OpenMP or HIP device state beyond selecting its own device.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   - StiffAlgo::CalcGlobalStiffness(FemData*)   src/FEM/StiffAlgo.h:46
+ *     (GPU strategies StiffAlgoGPU / StiffAlgoGpuOmp,
+ *      src/FEM/StiffAlgoGPU.cc:49-178, src/FEM/StiffAlgoGpuOmp.cc:49-204)
+ *       -> gf_mesh_upload + gf_ell_symbolic + gf_assemble
+ *   - FEM::ApplyConstraint(PEN)                   src/FEM/fem.cc:112-145
+ *       -> gf_apply_penalty
+ *   - ELLmatrix::AxyGPU                           src/SPRmatrix/ELLmatrix.cc:209-285
+ *       -> gf_spmv
+ *   - ELLmatrix::CG / SolveCgGpu                  src/SPRmatrix/ELLmatrix.cc:289-459
+ *     (SPRmatrix::CPU_CG semantics, src/SPRmatrix/SPRmatrix.cc:405-480)
+ *       -> gf_pcg
+ *   - ELL arrays m_matdata/m_colidx/m_rownnz      src/SPRmatrix/ELLmatrix.h:82-90
+ *       -> gf_ell_download / gf_ell_upload
+ *   - femColor::MakeGreedyColoring                src/FEM/femColor.cc:132-191
+ *       -> computed inside gf_ell_symbolic, read back by gf_ell_colors
+ */
+#ifndef GPUFEM_H_
+#define GPUFEM_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+#define GF_OK 0
+#define GF_ERR_ARG (-1)         /* bad argument (null, out of range, mismatch) */
+#define GF_ERR_HIP (-2)         /* a HIP runtime call failed */
+#define GF_ERR_NOMEM (-3)       /* host or device allocation failed */
+#define GF_ERR_UNSUPPORTED (-4) /* element type / rule / size not supported */
+#define GF_ERR_DEGENERATE (-5)  /* an element has detJ <= 0 at a Gauss point */
+#define GF_ERR_BREAKDOWN (-6)   /* solver breakdown (p.q <= 0 or NaN) */
+#define GF_ERR_NOCONV (-7)      /* maxit reached before the tolerance */
+
+/* ---- element types (reference node orderings, src/FEM/StiffAlgo.cc:47-202) */
+#define GF_Q4 1
+#define GF_Q8 2
+#define GF_BRICK8 3
+#define GF_BRICK20 4
+
+/* ---- opaque handles ------------------------------------------------------ */
+typedef struct gf_ctx gf_ctx;
+typedef struct gf_mesh gf_mesh;
+typedef struct gf_ell gf_ell;
+
+/* Device-side timing of one call, measured with HIP events recorded on the
+ * context's stream around the call's kernels. */
+typedef struct gf_timing {
+  double ms_device;  /* event-to-event time on the stream (kernels only) */
+  double ms_host;    /* host wall time of the call (includes copies) */
+  int64_t launches;  /* kernel launches issued */
+} gf_timing;
+
+/* ======================================================================== */
+/* Structured synthetic meshes (host only; no GPU needed).                  */
+/* Nodes are numbered lexicographically (x fastest, then y, then z) over    */
+/* the points the element type uses; connectivity is 1-based in the         */
+/* reference node ordering; DOF = dim*(node-1)+c.                           */
+/* No reference counterpart: the reference reads neutral files              */
+/* (src/Utils/fileIO.cc:73-404); these grids stand in for them.             */
+/* ======================================================================== */
+typedef struct gf_grid_desc {
+  int32_t elem_type; /* GF_Q4 .. GF_BRICK20 */
+  int32_t nx, ny, nz; /* elements per axis (nz ignored in 2D) */
+  double h;           /* element edge length */
+} gf_grid_desc;
+
+int gf_grid_sizes(const gf_grid_desc* g, int64_t* nnodes, int64_t* nelem, int32_t* nen,
+                  int32_t* dim);
+/* conn: [nelem][nen] 1-based; coords: [nnodes][dim] */
+int gf_grid_fill(const gf_grid_desc* g, int32_t* conn, double* coords);
+/* Nodes (1-based, ascending) on a boundary face: 0:x=0 1:x=max 2:y=0 3:y=max
+ * 4:z=0 5:z=max.  Pass nodes = NULL to get the count.  Returns the count or
+ * a negative error. */
+int64_t gf_grid_face_nodes(const gf_grid_desc* g, int32_t face, int32_t* nodes);
+/* Adds the consistent nodal loads of a uniform traction q (force per unit
+ * edge length in 2D, per unit area in 3D) in direction comp (0:x 1:y 2:z)
+ * on a boundary face into f[nnodes*dim]. */
+int gf_grid_face_load(const gf_grid_desc* g, int32_t face, int32_t comp, double q,
+                      double* f);
+/* Deterministic streams (SplitMix64 + Box-Muller). */
+int gf_rng_normal(uint64_t seed, int64_t n, double* out);
+int gf_rng_uniform_int(uint64_t seed, int64_t n, int32_t lo, int32_t hi, int32_t* out);
+
+/* ======================================================================== */
+/* Context                                                                   */
+/* ======================================================================== */
+int gf_create(gf_ctx** out, int32_t device);
+int gf_destroy(gf_ctx* ctx);
+/* Message of the last failure on ctx (ctx may be NULL: last global error). */
+const char* gf_last_error(const gf_ctx* ctx);
+/* Compute units and device name of the context's GPU. */
+int gf_device_info(gf_ctx* ctx, int32_t* num_cu, char* name, int32_t name_len);
+
+/* ======================================================================== */
+/* Mesh on the device.  Replaces the FemData arrays the GPU strategies read  */
+/* (src/FEM/FemData.h:79-105; H2D at src/FEM/StiffAlgoGpuOmp.cc:105-125).    */
+/* ======================================================================== */
+typedef struct gf_mesh_desc {
+  int32_t elem_type;    /* GF_Q4 .. GF_BRICK20 */
+  int32_t ngauss;       /* Gauss points per direction: 2 or 3 */
+  int64_t nnodes, nelem;
+  const int32_t* conn;  /* [nelem][nen], 1-based */
+  const double* coords; /* [nnodes][dim] */
+  const double* elem_E; /* [nelem] Young's modulus per element, or NULL */
+  double E, nu;         /* E used when elem_E is NULL */
+} gf_mesh_desc;
+
+int gf_mesh_upload(gf_ctx* ctx, const gf_mesh_desc* desc, gf_mesh** out);
+int gf_mesh_free(gf_mesh* mesh);
+
+/* ======================================================================== */
+/* ELL matrix.  Slot-major ("column-wise") ELLPACK as the reference's       */
+/* ELLmatrix (src/SPRmatrix/ELLmatrix.h:35-44): val[row + slot*n],          */
+/* col[row + slot*n] ascending within a row, rownnz[n]; width = max rownnz. */
+/* Padding slots hold val 0 and col = row.                                  */
+/* ======================================================================== */
+typedef struct gf_ell_info {
+  int64_t n;       /* rows (= DOF) */
+  int32_t width;   /* slots per row */
+  int32_t ncolors; /* element colors (0 for a matrix made by gf_ell_upload) */
+  int64_t nnz;     /* sum of rownnz */
+  int64_t bytes;   /* device bytes held by the matrix */
+} gf_ell_info;
+
+/* Symbolic structure (node adjacency x dim^2, sorted), greedy element
+ * coloring and the element -> slot map, built once per mesh. */
+int gf_ell_symbolic(gf_ctx* ctx, const gf_mesh* mesh, gf_ell** out, gf_timing* t);
+int gf_ell_info_get(const gf_ell* m, gf_ell_info* info);
+/* elem_color: [nelem], colors 1..ncolors */
+int gf_ell_colors(gf_ctx* ctx, const gf_ell* m, int32_t* elem_color);
+int gf_ell_free(gf_ell* m);
+/* Matrix from host arrays (slot-major, width slots). */
+int gf_ell_upload(gf_ctx* ctx, int64_t n, int32_t width, const double* val,
+                  const int32_t* col, const int32_t* rownnz, gf_ell** out);
+/* val, col: [width*n]; rownnz: [n] (any may be NULL to skip). */
+int gf_ell_download(gf_ctx* ctx, const gf_ell* m, double* val, int32_t* col,
+                    int32_t* rownnz);
+
+/* Numeric assembly: Ke by Gauss quadrature fused with the colored,
+ * atomic-free scatter (one launch per color).  Values are zeroed first. */
+int gf_assemble(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, gf_timing* t);
+
+/* Penalty constraint (FEM::ApplyConstraint PEN): pen = coef * max_i K_ii,
+ * then K_dd <- pen * K_dd for each listed DOF (0-based; a DOF listed k
+ * times is multiplied k times, as repeated SetElem calls would).  *pen_out
+ * may be NULL. */
+int gf_apply_penalty(gf_ctx* ctx, gf_ell* m, const int32_t* dofs, int64_t ndofs,
+                     double coef, double* pen_out);
+
+/* Y = K X for nrhs right-hand sides stored column after column
+ * (x[k*n + i]).  Host pointers. */
+int gf_spmv(gf_ctx* ctx, const gf_ell* m, const double* x, double* y, int32_t nrhs,
+            gf_timing* t);
+
+/* ======================================================================== */
+/* Solver                                                                    */
+/* ======================================================================== */
+#define GF_PRECOND_NONE 0   /* reference CG (src/SPRmatrix/SPRmatrix.cc:405-480) */
+#define GF_PRECOND_JACOBI 1 /* Jacobi-PCG (Saad Alg. 9.1, M = diag K) */
+
+#define GF_PREC_FP64 0  /* everything in FP64 */
+#define GF_PREC_MIXED 1 /* FP32 inner PCG inside FP64 iterative refinement */
+
+#define GF_SCHED_AUTO 0
+#define GF_SCHED_KERNELS 1    /* one launch per BLAS step, host-free loop */
+#define GF_SCHED_GRAPH 2      /* iteration chunks captured in a hipGraph */
+#define GF_SCHED_PERSISTENT 3 /* one cooperative kernel, grid barriers */
+
+typedef struct gf_solve_opts {
+  int32_t precond;   /* GF_PRECOND_* */
+  int32_t precision; /* GF_PREC_* */
+  int32_t schedule;  /* GF_SCHED_* */
+  int32_t use_x0;    /* 1: x holds the initial guess; 0: start from zero */
+  int32_t maxit;     /* iteration cap (per right-hand side) */
+  int32_t device_ptrs; /* 1: b and x are device pointers */
+  double rtol;       /* stop when ||b - K x||_2 <= rtol ||b||_2 */
+  double inner_rtol; /* GF_PREC_MIXED: inner reduction per refinement sweep */
+} gf_solve_opts;
+
+typedef struct gf_solve_stats {
+  int32_t iterations;   /* inner iterations (max over right-hand sides) */
+  int32_t refinements;  /* outer FP64 sweeps (GF_PREC_MIXED) */
+  int32_t converged;    /* 1 if every right-hand side met rtol */
+  int32_t status;       /* GF_OK, GF_ERR_NOCONV or GF_ERR_BREAKDOWN */
+  double relres;        /* max over RHS of recursive ||r||/||b|| at exit */
+  double true_relres;   /* max over RHS of ||b - K x||/||b|| recomputed in FP64 */
+  double ms_device;     /* event time of the solve on the stream */
+  double ms_host;       /* host wall time of the call */
+} gf_solve_stats;
+
+void gf_solve_opts_default(gf_solve_opts* o);
+/* Solves K X = B for nrhs right-hand sides (column after column). */
+int gf_pcg(gf_ctx* ctx, gf_ell* m, const double* b, double* x, int32_t nrhs,
+           const gf_solve_opts* opts, gf_solve_stats* stats);
+
+/* Average duration (ms) of the SpMV kernel the solver runs each iteration
+ * (q = K p fused with the p.q partial sums), timed with HIP events over nrep
+ * back-to-back launches on the context's stream.  No reference counterpart:
+ * it mirrors the per-kernel event timing of OCLwrapper
+ * (src/OpenCL/OCLwrapper.cc:400-434) for the roofline figure. */
+int gf_profile_spmv(gf_ctx* ctx, gf_ell* m, int32_t nrep, double* ms_avg);
+
+/* Device memory helpers for callers that keep vectors resident in HBM. */
+int gf_dev_alloc(gf_ctx* ctx, int64_t bytes, void** dptr);
+int gf_dev_free(gf_ctx* ctx, void* dptr);
+int gf_copy_h2d(gf_ctx* ctx, void* dst, const void* src, int64_t bytes);
+int gf_copy_d2h(gf_ctx* ctx, void* dst, const void* src, int64_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPUFEM_H_ */

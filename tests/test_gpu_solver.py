@@ -1,0 +1,183 @@
+"""GPU parity of SpMV and the (Jacobi-preconditioned) CG against the
+reference's known answers and the FP64 oracle."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import gpufem_amd as gf
+import oracle as O
+from gpufem_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
+
+
+def upload_oracle(ctx, m):
+    val, col, rn = m.export()
+    return ctx.upload_ell(val, col, rn)
+
+
+def diag_corner(n):
+    m = O.EllMatrix(n)
+    for i in range(n):
+        m.add(i, i, float((i + 1) ** 2))
+    m.set(0, 0, 2)
+    m.set(0, 1, 2)
+    m.set(1, 0, 2)
+    return m
+
+
+# ---- SpMV known answers (src/Test/AxyGPUTest.cpp:44-197) ------------------------------
+@pytest.mark.parametrize("n", GOLD["spmv_diagonal"]["sizes"])
+def test_spmv_diagonal(gpu_ctx, n):
+    m = O.EllMatrix(n)
+    for i in range(n):
+        m.add(i, i, i + 1.0)
+    K = upload_oracle(gpu_ctx, m)
+    y, _ = gpu_ctx.spmv(K, np.arange(1, n + 1, dtype=float))
+    assert np.array_equal(y, np.arange(1, n + 1, dtype=float) ** 2)
+
+
+@pytest.mark.parametrize("n", GOLD["spmv_band_ones"]["sizes"])
+def test_spmv_band(gpu_ctx, n):
+    val = np.ones((8, n))
+    col = np.tile(np.arange(8, dtype=np.int32)[:, None], (1, n))
+    K = gpu_ctx.upload_ell(val, col, np.full(n, 8, dtype=np.int32))
+    y, _ = gpu_ctx.spmv(K, np.full(n, 2.0))
+    assert np.all(y == 16.0)
+
+
+def test_spmv_shifted_band(gpu_ctx):
+    m = O.EllMatrix(64)
+    for i in range(64):
+        for j in range(4):
+            m.add(i, i + j, 1)
+    K = upload_oracle(gpu_ctx, m)
+    y, _ = gpu_ctx.spmv(K, np.full(64, 2.0))
+    assert np.array_equal(y, m.axy(np.full(64, 2.0)))
+
+
+def test_spmv_random_matches_oracle_and_multi_rhs(gpu_ctx):
+    g = gf.Grid(gf.GF_Q8, 20, 10)
+    conn, coords = g.fill()
+    Ko, _ = O.assemble(2, 3, conn, coords, 210000.0, 0.3, skip_zero=False)
+    K = upload_oracle(gpu_ctx, Ko)
+    rng = np.random.default_rng(0)
+    X = rng.normal(size=(3, g.ndof))
+    Y, _ = gpu_ctx.spmv(K, X)
+    for k in range(3):
+        # same slot order as the oracle's Axy, so FP64 results agree to rounding
+        yo = Ko.axy(X[k])
+        assert np.abs(Y[k] - yo).max() <= 1e-13 * np.abs(yo).max()
+
+
+# ---- CG known answers (src/Test/AxyGPUTest.cpp:201-367) -------------------------------
+@pytest.mark.parametrize("n", GOLD["cg_diag_corner"]["sizes"])
+@pytest.mark.parametrize("sched", [gf.GF_SCHED_KERNELS, gf.GF_SCHED_GRAPH])
+def test_cg_kat_diag_corner(gpu_ctx, n, sched):
+    K = upload_oracle(gpu_ctx, diag_corner(n))
+    b = np.arange(1, n + 1, dtype=float) ** 2
+    x, st = gpu_ctx.pcg(K, b, precond=gf.GF_PRECOND_NONE, rtol=1e-6, maxit=500, schedule=sched)
+    assert abs(x[0] + 1) < 1e-3 and abs(x[1] - 1.5) < 1e-3 and np.all(np.abs(x[2:] - 1) < 1e-3)
+    x, st = gpu_ctx.pcg(K, b, rtol=1e-12, schedule=sched)
+    assert abs(x[0] + 1) < 1e-9 and abs(x[1] - 1.5) < 1e-9 and st.converged
+
+
+def test_cg_kat_tridiag(gpu_ctx):
+    m = O.EllMatrix(4)
+    for i in range(4):
+        m.add(i, i, 2)
+        m.add(i, i + 1, -1)
+        m.add(i, i - 1, -1)
+    K = upload_oracle(gpu_ctx, m)
+    x, st = gpu_ctx.pcg(K, np.ones(4), precond=gf.GF_PRECOND_NONE, rtol=1e-5, maxit=200)
+    assert np.allclose(x, GOLD["cg_tridiag_4"]["x"], atol=1e-3)
+
+
+def test_cg_kat_ones_plus_identity(gpu_ctx):
+    n = 32
+    val = np.ones((n, n))
+    col = np.tile(np.arange(n, dtype=np.int32)[:, None], (1, n))
+    for i in range(n):
+        val[i, i] = 2.0
+    K = gpu_ctx.upload_ell(val, col, np.full(n, n, dtype=np.int32))
+    x, st = gpu_ctx.pcg(K, np.ones(n), precond=gf.GF_PRECOND_NONE, rtol=1e-5, maxit=500)
+    assert np.allclose(x, GOLD["cg_ones_plus_identity_32"]["x_each"], atol=1e-3)
+
+
+def test_reference_cg_iterations_match_oracle(gpu_ctx):
+    """Unpreconditioned CG with the reference's stopping rule (r.r <= eps^2 r0.r0)."""
+    p = P.config_q4_plate()
+    Ko, _ = O.assemble(2, 2, p.conn, p.coords, p.E, p.nu, skip_zero=False)
+    O.apply_penalty(Ko, 2, p.supports)
+    xo, ito, _ = O.cg(Ko, p.f, 100000, 1e-8)
+    K = upload_oracle(gpu_ctx, Ko)
+    x, st = gpu_ctx.pcg(K, p.f, precond=gf.GF_PRECOND_NONE, rtol=1e-8)
+    assert abs(st.iterations - ito) <= max(3, ito // 50)
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+# ---- the assembled problem: plate 64x32, Jacobi-PCG to 1e-8 ----------------------------
+@pytest.mark.parametrize("t,nx,ny,nz", [(gf.GF_Q4, 64, 32, 0), (gf.GF_Q8, 32, 16, 0),
+                                        (gf.GF_BRICK8, 8, 8, 8), (gf.GF_BRICK20, 4, 4, 4)])
+def test_pcg_parity_assembled(gpu_ctx, t, nx, ny, nz):
+    p = P.structured(t, nx, ny, nz)
+    mesh = gpu_ctx.upload_mesh(p.elem_type, p.ngauss, p.conn, p.coords, p.E, p.nu)
+    K = gpu_ctx.symbolic(mesh)
+    gpu_ctx.assemble(mesh, K)
+    gpu_ctx.apply_penalty(K, p.fixed_dofs, 1e4)
+    x, st = gpu_ctx.pcg(K, p.f, rtol=1e-8)
+    Ko, _ = O.assemble(p.dim, p.ngauss, p.conn, p.coords, p.E, p.nu, skip_zero=False)
+    O.apply_penalty(Ko, p.dim, p.supports)
+    xo, ito, relo, _ = O.pcg_jacobi(Ko, p.f, rtol=1e-8)
+    assert st.converged and st.true_relres <= 1.2e-8
+    assert abs(st.iterations - ito) <= max(2, ito // 100)
+    # both stop at 1e-8 relative residual; answers agree far below that
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+def test_pcg_schedules_bitwise_identical(gpu_ctx):
+    p = P.config_q4_plate()
+    mesh = gpu_ctx.upload_mesh(p.elem_type, p.ngauss, p.conn, p.coords, p.E, p.nu)
+    K = gpu_ctx.symbolic(mesh)
+    gpu_ctx.assemble(mesh, K)
+    gpu_ctx.apply_penalty(K, p.fixed_dofs)
+    x1, s1 = gpu_ctx.pcg(K, p.f, schedule=gf.GF_SCHED_KERNELS)
+    x2, s2 = gpu_ctx.pcg(K, p.f, schedule=gf.GF_SCHED_GRAPH)
+    x3, s3 = gpu_ctx.pcg(K, p.f, schedule=gf.GF_SCHED_GRAPH)
+    assert s1.iterations == s2.iterations == s3.iterations
+    assert np.array_equal(x1, x2) and np.array_equal(x2, x3)
+
+
+def test_pcg_multi_rhs_and_warm_start(gpu_ctx):
+    p = P.structured(gf.GF_Q4, 24, 12)
+    mesh = gpu_ctx.upload_mesh(p.elem_type, p.ngauss, p.conn, p.coords, p.E, p.nu)
+    K = gpu_ctx.symbolic(mesh)
+    gpu_ctx.assemble(mesh, K)
+    gpu_ctx.apply_penalty(K, p.fixed_dofs)
+    F = P.random_top_loads(p, 3, 99)
+    X, st = gpu_ctx.pcg(K, F)
+    for k in range(3):
+        xk, sk = gpu_ctx.pcg(K, F[k])
+        assert np.array_equal(X[k], xk)
+    xw, sw = gpu_ctx.pcg(K, F[0], x0=X[0])
+    assert sw.iterations == 0
+
+
+def test_pcg_error_paths(gpu_ctx):
+    K = upload_oracle(gpu_ctx, diag_corner(32))
+    b = np.arange(1, 33, dtype=float) ** 2
+    x, st = gpu_ctx.pcg(K, b, rtol=1e-14, maxit=2, raise_on_fail=False)
+    assert st.code == gf.GF_ERR_NOCONV and st.iterations == 2 and not st.converged
+    # indefinite matrix: p.Kp <= 0 is detected as breakdown (no fault, no hang)
+    n = 4
+    val = np.array([[1.0, -1.0, 1.0, 1.0]])
+    col = np.arange(n, dtype=np.int32)[None, :]
+    Kn = gpu_ctx.upload_ell(val, col, np.ones(n, dtype=np.int32))
+    x, st = gpu_ctx.pcg(Kn, np.array([0.0, 1.0, 0.0, 0.0]), precond=gf.GF_PRECOND_NONE,
+                        raise_on_fail=False)
+    assert st.code == gf.GF_ERR_BREAKDOWN
+    x, st = gpu_ctx.pcg(K, np.zeros(32))
+    assert st.iterations == 0 and np.all(x == 0)

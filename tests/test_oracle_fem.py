@@ -1,0 +1,198 @@
+"""Pins the oracle's mesh-level functions: greedy coloring, colored assembly,
+penalty constraint and force vector, against brute force on small meshes and
+closed forms; and the structured mesh generator the benchmarks use."""
+import numpy as np
+import pytest
+
+import gpufem_amd as gf
+import oracle as O
+from gpufem_amd import problems as P
+
+CASES = [(gf.GF_Q4, 6, 4, 0, 2), (gf.GF_Q8, 5, 3, 0, 3), (gf.GF_BRICK8, 3, 3, 2, 2),
+         (gf.GF_BRICK20, 2, 2, 2, 3)]
+
+
+def brute_first_fit(conn):
+    nel = conn.shape[0]
+    sets = [set(c) for c in conn.tolist()]
+    color = [0] * nel
+    for e in range(nel):
+        used = {color[f] for f in range(nel) if f != e and sets[e] & sets[f]}
+        c = 1
+        while c in used:
+            c += 1
+        color[e] = c
+    return np.array(color)
+
+
+@pytest.mark.parametrize("t,nx,ny,nz,ng", CASES)
+def test_coloring_first_fit_and_invariant(t, nx, ny, nz, ng):
+    g = gf.Grid(t, nx, ny, nz)
+    conn, coords = g.fill()
+    col, nc = O.greedy_coloring(conn, g.nnodes)
+    assert np.array_equal(col, brute_first_fit(conn))
+    for c in range(1, nc + 1):
+        nodes = conn[col == c].ravel()
+        assert nodes.size == np.unique(nodes).size      # no shared node within a color
+    assert nc == (4 if g.dim == 2 else 8)
+
+
+def test_coloring_shuffled_connectivity_matches_brute_force():
+    g = gf.Grid(gf.GF_Q4, 5, 4)
+    conn, _ = g.fill()
+    rng = np.random.default_rng(2)
+    conn = conn[rng.permutation(conn.shape[0])]
+    col, nc = O.greedy_coloring(conn, g.nnodes)
+    assert np.array_equal(col, brute_first_fit(conn))
+
+
+@pytest.mark.parametrize("t,nx,ny,nz,ng", CASES)
+def test_assembly_matches_dense_sum(t, nx, ny, nz, ng):
+    g = gf.Grid(t, nx, ny, nz)
+    conn, coords = g.fill()
+    rng = np.random.default_rng(4)
+    coords = coords + 0.05 * rng.uniform(-1, 1, coords.shape)     # distort
+    E = rng.uniform(1.0, 3.0, conn.shape[0])
+    K, _ = O.assemble(g.dim, ng, conn, coords, 1.0, 0.3, elem_E=E)
+    n = g.ndof
+    A = np.zeros((n, n))
+    for e in range(conn.shape[0]):
+        xe = coords[conn[e] - 1].T.copy()
+        Ke = O.element_stiffness(g.dim, g.nen, ng, xe, E[e], 0.3)
+        dofs = ((conn[e] - 1)[:, None] * g.dim + np.arange(g.dim)).ravel()
+        A[np.ix_(dofs, dofs)] += Ke
+    D = K.dense()
+    assert np.abs(D - A).max() <= 1e-12 * np.abs(A).max()
+    # rigid translations are in the null space of the unconstrained matrix
+    for c in range(g.dim):
+        u = np.zeros((g.nnodes, g.dim))
+        u[:, c] = 1
+        assert np.abs(D @ u.ravel()).max() <= 1e-11 * np.abs(A).max()
+    # structure: row of node v = dim * |adjacency(v)| columns, sorted
+    val, col, rn = K.export()
+    adj = [set() for _ in range(g.nnodes)]
+    for c in conn - 1:
+        for a in c:
+            adj[a].update(c.tolist())
+    for i in range(n):
+        v = i // g.dim
+        expect = sorted(g.dim * w + c for w in adj[v] for c in range(g.dim))
+        assert list(col[: rn[i], i]) == expect
+
+
+def test_assembly_thread_count_invariant():
+    g = gf.Grid(gf.GF_Q8, 12, 8)
+    conn, coords = g.fill()
+    K1, _ = O.assemble(2, 3, conn, coords, 210000.0, 0.3, nthreads=1)
+    K8, _ = O.assemble(2, 3, conn, coords, 210000.0, 0.3, nthreads=8)
+    v1, c1, r1 = K1.export()
+    v8, c8, r8 = K8.export()
+    assert np.array_equal(c1, c8) and np.array_equal(r1, r8) and np.array_equal(v1, v8)
+
+
+def test_q4_nnz_closed_form():
+    # Q4 nnz = 4(3a-2)(3b-2) for a x b nodes (structured grid)
+    for nx, ny in [(4, 3), (64, 32)]:
+        g = gf.Grid(gf.GF_Q4, nx, ny)
+        conn, coords = g.fill()
+        K, _ = O.assemble(2, 2, conn, coords, 210000.0, 0.3)
+        a, b = nx + 1, ny + 1
+        assert K.nnz() == 4 * (3 * a - 2) * (3 * b - 2)
+
+
+def test_brick8_nnz_closed_form():
+    g = gf.Grid(gf.GF_BRICK8, 3, 4, 2)
+    conn, coords = g.fill()
+    K, _ = O.assemble(3, 2, conn, coords, 210000.0, 0.3)
+    a, b, c = 4, 5, 3
+    assert K.nnz() == 9 * (3 * a - 2) * (3 * b - 2) * (3 * c - 2)
+
+
+def test_zero_skip_only_removes_exact_zeros():
+    g = gf.Grid(gf.GF_Q8, 4, 3)
+    conn, coords = g.fill()
+    Ks, _ = O.assemble(2, 3, conn, coords, 210000.0, 0.3, skip_zero=True)
+    Kf, _ = O.assemble(2, 3, conn, coords, 210000.0, 0.3, skip_zero=False)
+    assert Ks.nnz() <= Kf.nnz()
+    assert np.array_equal(Ks.dense(), Kf.dense())
+
+
+def test_penalty_closed_form():
+    m = O.EllMatrix(6)
+    for i in range(6):
+        m.add(i, i, float(i + 1))
+        if i:
+            m.add(i, i - 1, -0.5)
+    sup = np.array([[2, 1, 0], [3, 0, 1]], dtype=np.int32)      # node 2 x, node 3 y
+    pen = O.apply_penalty(m, 2, sup, 1e4)
+    assert pen == 6.0 * 1e4
+    D = m.dense()
+    assert D[2, 2] == 3.0 * 6e4 and D[5, 5] == 6.0 * 6e4
+    assert D[0, 0] == 1.0 and D[3, 3] == 4.0 and D[1, 0] == -0.5
+
+
+def test_force_vector_assignment():
+    loads = np.array([[2, 1.5, 0.0], [1, 0.0, -2.0], [2, 0.0, 7.0]])
+    f = O.force_vector(2, 3, loads)
+    # later rows overwrite earlier ones; zero components are skipped
+    assert list(f) == [0.0, -2.0, 1.5, 7.0, 0.0, 0.0]
+
+
+# ---- structured mesh generator ---------------------------------------------------------
+@pytest.mark.parametrize("t,nx,ny,nz,ng", CASES)
+def test_grid_positive_jacobian_and_counts(t, nx, ny, nz, ng):
+    g = gf.Grid(t, nx, ny, nz)
+    conn, coords = g.fill()
+    assert conn.min() == 1 and conn.max() == g.nnodes
+    assert np.unique(conn).size == g.nnodes
+    for e in range(conn.shape[0]):
+        xe = coords[conn[e] - 1]
+        xi = O.node_natural_coords(g.dim, g.nen)
+        # nodes sit at the affine image of their natural coordinates
+        lo = xe.min(axis=0)
+        assert np.allclose(xe, lo + (xi + 1) / 2, atol=1e-14)
+
+
+def test_survey_sizes():
+    expect = {(gf.GF_Q4, 64, 32, 0): (2145, 2048), (gf.GF_Q8, 1024, 512, 0): (1575937, 524288),
+              (gf.GF_BRICK8, 128, 128, 128): (2146689, 2097152),
+              (gf.GF_BRICK20, 48, 48, 48): (463393, 110592),
+              (gf.GF_BRICK8, 64, 32, 32): (70785, 65536)}
+    for (t, nx, ny, nz), (nn, ne) in expect.items():
+        g = gf.Grid(t, nx, ny, nz)
+        assert (g.nnodes, g.nelem) == (nn, ne)
+
+
+@pytest.mark.parametrize("t", [gf.GF_Q4, gf.GF_Q8, gf.GF_BRICK8, gf.GF_BRICK20])
+def test_face_loads_sum_to_total_force(t):
+    g = gf.Grid(t, 5, 4, 3)
+    if g.dim == 2:
+        f = g.face_load(1, 1, -1.0)
+        assert abs(f.sum() + 4.0) < 1e-12                      # q * edge length
+    else:
+        f = g.face_load(5, 2, -1.0)
+        assert abs(f.sum() + 20.0) < 1e-12                     # q * face area
+    # every loaded node is on the face
+    face = g.face_nodes(1 if g.dim == 2 else 5)
+    loaded = np.nonzero(f.reshape(-1, g.dim).any(axis=1))[0] + 1
+    assert set(loaded.tolist()) <= set(face.tolist())
+
+
+def test_q4_plate_config_matches_survey():
+    p = P.config_q4_plate()
+    assert p.ndof == 4290 and p.conn.shape == (2048, 4)
+    assert p.fixed_dofs.size == 66
+    f = p.f.reshape(-1, 2)
+    right = p.grid.face_nodes(1) - 1
+    assert np.allclose(f[right, 1][1:-1], -1.0) and np.allclose(f[right, 1][[0, -1]], -0.5)
+    assert np.allclose(O.force_vector(2, p.grid.nnodes, p.loads), p.f)
+
+
+def test_rng_streams_deterministic():
+    a = gf.rng_normal(0x5EED0003, 100001)
+    b = gf.rng_normal(0x5EED0003, 100001)
+    assert np.array_equal(a, b)
+    assert abs(a.mean()) < 0.01 and abs(a.std() - 1) < 0.01
+    u = gf.rng_uniform_int(0x5EED0005, 10000, 16, 256)
+    assert u.min() >= 16 and u.max() <= 256
+    assert np.array_equal(u, gf.rng_uniform_int(0x5EED0005, 10000, 16, 256))
