@@ -66,7 +66,7 @@ def cpu_baseline(p, gpu_iters, budget_s):
     # numeric re-assembly: structure already present
     O.load().orc_ell_zero_values(K.m)
     _, t_num = O.assemble(p.dim, p.ngauss, p.conn, p.coords, p.E, p.nu, elem_E=p.elem_E,
-                          skip_zero=False, nthreads=threads)
+                          skip_zero=False, nthreads=threads, into=K)
     O.apply_penalty(K, p.dim, p.supports)
     # bounded PCG sample: iterations until about half the remaining budget is spent
     spent = time.perf_counter() - t0
@@ -99,6 +99,10 @@ def main():
     ap.add_argument("--workload", default="q8_plate_1024x512")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=25.0)
+    ap.add_argument("--schedule", default="persistent",
+                    choices=["kernels", "graph", "persistent"])
+    ap.add_argument("--compare", default="all", choices=["all", "none"],
+                    help="also time the other FP64 schedules and the mixed solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: --gpus %d is not supported (the path would "
@@ -119,6 +123,11 @@ def main():
     K = ctx.symbolic(mesh)
     t_sym = time.perf_counter() - t0
     log("symbolic: width %d, nnz %d, %d colors, %.3f s" % (K.width, K.nnz, K.ncolors, t_sym))
+    ctx.assemble(mesh, K)
+    variant, variant_ms = ctx.tune(K)
+    log("spmv variants (ms): %s -> %d" % (" ".join("%.4f" % v for v in variant_ms), variant))
+    sched = {"kernels": gf.GF_SCHED_KERNELS, "graph": gf.GF_SCHED_GRAPH,
+             "persistent": gf.GF_SCHED_PERSISTENT}[args.schedule]
 
     # inputs resident in HBM
     vb = n * 8
@@ -127,16 +136,19 @@ def main():
     gf.check(lib.gf_dev_alloc(ctx.handle, vb, C.byref(d_x)), ctx.handle)
     f = np.ascontiguousarray(p.f)
     gf.check(lib.gf_copy_h2d(ctx.handle, d_b, f.ctypes.data_as(C.c_void_p), vb), ctx.handle)
-    opts = gf.solve_opts(device_ptrs=1, rtol=1e-8)
+    opts = gf.solve_opts(device_ptrs=1, rtol=1e-8, schedule=sched)
     dofs = np.ascontiguousarray(p.fixed_dofs)
+
+    def solve(o):
+        st = gf._lib.SolveStats()
+        gf.check(lib.gf_pcg(ctx.handle, K.handle, d_b, d_x, 1, C.byref(o), C.byref(st)),
+                 ctx.handle)
+        return st
 
     def step():
         ta = ctx.assemble(mesh, K)
         ctx.apply_penalty(K, dofs)
-        st = gf._lib.SolveStats()
-        gf.check(lib.gf_pcg(ctx.handle, K.handle, d_b, d_x, 1, C.byref(opts), C.byref(st)),
-                 ctx.handle)
-        return ta, st
+        return ta, solve(opts)
 
     for i in range(args.warmup):
         ta, st = step()
@@ -157,6 +169,26 @@ def main():
     t_asm = float(np.mean(asm_ms)) / 1e3
     value = n / t_asm
 
+    solves = {"fp64_" + args.schedule: {
+        "iterations": float(np.mean(iters)), "seconds": float(np.mean(pcg_ms)) / 1e3,
+        "true_relres": float(np.max(relres))}}
+    if args.compare == "all":
+        for name, o in [
+                ("fp64_graph", gf.solve_opts(device_ptrs=1, rtol=1e-8,
+                                             schedule=gf.GF_SCHED_GRAPH)),
+                ("fp64_kernels", gf.solve_opts(device_ptrs=1, rtol=1e-8,
+                                               schedule=gf.GF_SCHED_KERNELS)),
+                ("fp64_persistent", gf.solve_opts(device_ptrs=1, rtol=1e-8,
+                                                  schedule=gf.GF_SCHED_PERSISTENT)),
+                ("mixed", gf.solve_opts(device_ptrs=1, rtol=1e-8,
+                                        precision=gf.GF_PREC_MIXED))]:
+            if name in solves:
+                continue
+            st = solve(o)
+            solves[name] = {"iterations": st.iterations, "seconds": st.ms_device / 1e3,
+                            "true_relres": st.true_relres, "refinements": st.refinements}
+            log("%s: %d it, %.3f s, true relres %.2e" % (name, st.iterations,
+                                                         st.ms_device / 1e3, st.true_relres))
     # roofline: the solver's SpMV kernel (q = K p with the p.q partials)
     ms_spmv = C.c_double()
     gf.check(lib.gf_profile_spmv(ctx.handle, K.handle, 50, C.byref(ms_spmv)), ctx.handle)
@@ -193,11 +225,9 @@ def main():
             "gflops_dense_convention": asm_flops / t_asm / 1e9,
             "fp64_peak_frac": asm_flops / t_asm / 1e12 / FP64_PEAK_TFLOPS,
         },
-        "solve": {
-            "fp64": {"iterations": it_mean, "seconds": pcg_s,
-                     "true_relres_max": float(np.max(relres)),
-                     "gbs_per_iteration": pcg_iter_bytes * it_mean / pcg_s / 1e9},
-        },
+        "solve": dict(solves, **{
+            "spmv_variant": variant, "spmv_variant_ms": [float(v) for v in variant_ms],
+            "fp64_gbs_per_iteration": pcg_iter_bytes * it_mean / pcg_s / 1e9}),
         "roofline": {
             "kernel": "k_spmv_pq",
             "bound": "hbm",

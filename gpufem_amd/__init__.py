@@ -226,6 +226,19 @@ class Context:
                                      C.byref(t)))
         return y, t
 
+    def tune(self, ell, force_variant=-1):
+        """Pick (or force) the SpMV variant; returns (chosen, ms per variant)."""
+        ms = np.zeros(5)
+        chosen = C.c_int32()
+        self._check(self.lib.gf_ell_tune(self.handle, ell.handle, force_variant, _ptr(ms),
+                                         C.byref(chosen)))
+        return chosen.value, ms
+
+    def profile_spmv(self, ell, nrep=20):
+        ms = C.c_double()
+        self._check(self.lib.gf_profile_spmv(self.handle, ell.handle, nrep, C.byref(ms)))
+        return ms.value
+
     def pcg(self, ell, b, x0=None, raise_on_fail=True, **opts):
         """Solve K x = b; b is (n,) or (nrhs, n).  Returns (x, SolveStats)."""
         b = np.ascontiguousarray(b, dtype=np.float64)

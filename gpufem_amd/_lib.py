@@ -87,6 +87,7 @@ SIGNATURES = {
     "gf_spmv": (C.c_int, [_P, _P, _P, _P, _I32, C.POINTER(Timing)]),
     "gf_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
     "gf_pcg": (C.c_int, [_P, _P, _P, _P, _I32, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "gf_ell_tune": (C.c_int, [_P, _P, _I32, _P, _P]),
     "gf_profile_spmv": (C.c_int, [_P, _P, _I32, C.POINTER(C.c_double)]),
     "gf_dev_alloc": (C.c_int, [_P, _I64, C.POINTER(_P)]),
     "gf_dev_free": (C.c_int, [_P, _P]),

@@ -45,9 +45,16 @@ hipError_t launch_penalty_apply(hipStream_t s, double* val, const int32_t* diag,
                                 const double* pen);
 
 // ---- SpMV / solver -----------------------------------------------------------------
-hipError_t launch_spmv(hipStream_t s, int64_t n, int32_t width, const double* val,
-                       const int32_t* col, const int32_t* rownnz, const double* x,
-                       double* y, int nrhs);
+// SpMV variants (see solver.hip)
+enum SpmvVariant { kThreadU1 = 0, kThreadU4 = 1, kThreadU8 = 2, kTile2 = 3, kTile4 = 4 };
+constexpr int kNumSpmvVariants = 5;
+constexpr int kMaxPcgBlocks = 2048;
+
+// workgroups of the standalone SpMV and the PCG kernels for n rows
+int spmv_blocks(int64_t n);
+hipError_t launch_spmv(hipStream_t s, int variant, int64_t n, const double* val,
+                       const int32_t* col, const int32_t* rownnz, const double* x, double* y,
+                       int nrhs);
 hipError_t launch_jacobi_inv(hipStream_t s, int64_t n, const double* val,
                              const int32_t* diag, double* dinv);
 
@@ -56,6 +63,7 @@ struct PcgState;  // device-resident scalars (solver.hip)
 struct PcgBuffers {
   int64_t n;
   int32_t width;
+  int variant;
   const double* val;
   const int32_t* col;
   const int32_t* rownnz;
@@ -67,14 +75,19 @@ struct PcgBuffers {
   double* p;
   double* q;
   double* part1;  // per-block partials of p.q
-  double* part2;  // per-block partials of (r.z, r.r)
-  int nblocks;    // grid of the vector kernels (<= max partials)
+  double* part2;  // per-block partials of (r.z, r.r[, b.b])
+  int nblocks;    // workgroups of every PCG kernel (<= kMaxPcgBlocks)
   PcgState* st;
 };
 size_t pcg_state_bytes();
+size_t grid_barrier_bytes();
 hipError_t launch_pcg_setup(hipStream_t s, const PcgBuffers& B, int use_x0, double rtol,
                             int maxit);
 hipError_t launch_pcg_iteration(hipStream_t s, const PcgBuffers& B, int parity);
+// co-resident workgroups for the persistent kernel of this variant (0: unavailable)
+int persistent_blocks(int variant, int num_cu);
+// all remaining iterations in one cooperative launch
+hipError_t launch_pcg_persistent(hipStream_t s, const PcgBuffers& B, void* barrier);
 // Host view of the scalar state.
 struct PcgHostState {
   int32_t it, done, status, pad;

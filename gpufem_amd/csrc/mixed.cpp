@@ -1,0 +1,106 @@
+// Host side of the mixed-precision solve (kernels in mixed.hip): FP32 copies
+// of the matrix values and of the Jacobi inverse, the FP32 work vectors, and
+// the chunked iteration loop.
+#include <string>
+
+#include "mixed.hpp"
+#include "solve_plan.hpp"
+
+struct gf_mixed_ws {
+  int64_t n = 0;
+  float *dinv32 = nullptr, *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr;
+  double* q64 = nullptr;
+  double* part3 = nullptr;
+  gf::MixedState* st = nullptr;
+};
+
+namespace gf {
+
+int set_error(gf_ctx* ctx, int code, const std::string& msg);
+
+namespace {
+constexpr int kChunk = 32;
+constexpr double kDelta = 0.1;  // refresh when the recursive residual fell 10x
+}  // namespace
+
+void release_mixed(gf_solve_plan* P) {
+  gf_mixed_ws* W = P->mixed;
+  if (!W) return;
+  float* f[] = {W->dinv32, W->r32, W->z32, W->p32, W->q32};
+  for (float* v : f)
+    if (v) (void)hipFree(v);
+  if (W->q64) (void)hipFree(W->q64);
+  if (W->part3) (void)hipFree(W->part3);
+  if (W->st) (void)hipFree(W->st);
+  delete W;
+  P->mixed = nullptr;
+}
+
+int run_mixed(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, PcgHostState* h,
+              int* refinements) {
+  gf_solve_plan* P = m->plan;
+  const int64_t n = m->n;
+  if (!P->mixed) {
+    gf_mixed_ws* W = new gf_mixed_ws();
+    P->mixed = W;
+    W->n = n;
+    const size_t fb = size_t(n) * sizeof(float);
+    float** f[] = {&W->dinv32, &W->r32, &W->z32, &W->p32, &W->q32};
+    hipError_t e = hipSuccess;
+    for (float** v : f)
+      if (e == hipSuccess) e = hipMalloc(v, fb);
+    if (e == hipSuccess) e = hipMalloc(&W->q64, size_t(n) * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&W->part3, 3 * kMaxPcgBlocks * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&W->st), mixed_state_bytes());
+    if (e != hipSuccess) {
+      release_mixed(P);
+      return set_error(ctx, GF_ERR_NOMEM, std::string("gf_pcg (mixed): workspace: ") + hipGetErrorString(e));
+    }
+  }
+  gf_mixed_ws* W = P->mixed;
+  const size_t slots = size_t(n) * size_t(m->width);
+  if (!m->d_val32) GF_HIP_CHECK(ctx, hipMalloc(&m->d_val32, slots * sizeof(float)));
+  if (!m->val32_valid) {
+    GF_HIP_CHECK(ctx, launch_to_f32(ctx->stream, int64_t(slots), m->d_val, m->d_val32));
+    GF_HIP_CHECK(ctx, launch_to_f32(ctx->stream, n, m->d_dinv, W->dinv32));
+    m->val32_valid = true;
+  }
+  MixedBuffers B;
+  B.n = n;
+  B.variant = P->variant;
+  B.nblocks = P->nblocks;
+  B.val = m->d_val;
+  B.col = m->d_col;
+  B.rownnz = m->d_rownnz;
+  B.b = P->wb;
+  B.x = P->wx;
+  B.q64 = W->q64;
+  B.val32 = m->d_val32;
+  B.dinv32 = W->dinv32;
+  B.r32 = W->r32;
+  B.z32 = W->z32;
+  B.p32 = W->p32;
+  B.q32 = W->q32;
+  B.part1 = P->part1;
+  B.part2 = P->part2;
+  B.part3 = W->part3;
+  B.st = W->st;
+  GF_HIP_CHECK(ctx, launch_mixed_setup(ctx->stream, B, opts.use_x0, opts.rtol, kDelta, opts.maxit));
+  MixedHostState hs{};
+  GF_HIP_CHECK(ctx, read_mixed_state(ctx->stream, B, &hs));
+  while (!hs.done) {
+    for (int it = 0; it < kChunk; ++it)
+      GF_HIP_CHECK(ctx, launch_mixed_iteration(ctx->stream, B, it & 1));
+    GF_HIP_CHECK(ctx, read_mixed_state(ctx->stream, B, &hs));
+  }
+  h->it = hs.it;
+  h->done = hs.done;
+  h->status = hs.status;
+  h->rr = hs.rr_true;
+  h->bnorm2 = hs.bnorm2;
+  h->rho = 0.0;
+  *refinements = hs.refinements;
+  return GF_OK;
+}
+
+}  // namespace gf

@@ -1,0 +1,52 @@
+// Mixed-precision Jacobi-PCG (FP32 inner iterations inside FP64 refinement).
+#pragma once
+
+#include "gf_internal.hpp"
+#include "launch.hpp"
+
+struct gf_solve_plan;
+
+namespace gf {
+
+struct MixedState;
+
+struct MixedBuffers {
+  int64_t n;
+  int variant;
+  int nblocks;
+  // FP64 outer system
+  const double* val;
+  const int32_t* col;
+  const int32_t* rownnz;
+  const double* b;
+  double* x;
+  double* q64;  // K x for the refreshed residual
+  // FP32 inner iteration
+  const float* val32;
+  const float* dinv32;
+  float* r32;
+  float* z32;
+  float* p32;
+  float* q32;
+  double* part1;  // p.q
+  double* part2;  // r.z, r.r of the update
+  double* part3;  // r.z, r.r of a refreshed residual
+  MixedState* st;
+};
+
+size_t mixed_state_bytes();
+hipError_t launch_to_f32(hipStream_t s, int64_t count, const double* in, float* out);
+hipError_t launch_mixed_setup(hipStream_t s, const MixedBuffers& B, int use_x0, double rtol,
+                              double delta, int maxit);
+hipError_t launch_mixed_iteration(hipStream_t s, const MixedBuffers& B, int parity);
+struct MixedHostState {
+  int32_t it, done, status, refinements;
+  double rr_true, rr_rec, bnorm2;
+};
+hipError_t read_mixed_state(hipStream_t s, const MixedBuffers& B, MixedHostState* h);
+
+int run_mixed(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, PcgHostState* h,
+              int* refinements);
+void release_mixed(gf_solve_plan* P);
+
+}  // namespace gf

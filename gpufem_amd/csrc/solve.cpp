@@ -1,9 +1,11 @@
-// Solver driver: gf_pcg.  Chooses the schedule, keeps the workspace on the
-// matrix, and runs each right-hand side through the device-resident PCG of
-// solver.hip.  Semantics follow SPRmatrix::CPU_CG
-// (src/SPRmatrix/SPRmatrix.cc:405-480) extended with the Jacobi
-// preconditioner: x0 = 0 (or the caller's guess), stop when
-// ||r||_2 <= rtol ||b||_2, report the true residual at exit.
+// Solver driver: gf_pcg, gf_ell_tune, gf_profile_spmv.
+//
+// Semantics follow SPRmatrix::CPU_CG (src/SPRmatrix/SPRmatrix.cc:405-480)
+// extended with the Jacobi preconditioner: x0 = 0 (or the caller's guess),
+// stop when ||r||_2 <= rtol ||b||_2, report the true residual at exit.  The
+// workspace (vectors, partials, scalar state, captured graphs) is cached on
+// the matrix; the SpMV variant is picked once per matrix by timing every
+// variant on the matrix itself.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -13,20 +15,8 @@
 
 #include "gf_internal.hpp"
 #include "launch.hpp"
+#include "mixed.hpp"
 #include "solve_plan.hpp"
-
-struct gf_solve_plan {
-  int64_t n = 0;
-  double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
-  double *wb = nullptr, *wx = nullptr;
-  double *part1 = nullptr, *part2 = nullptr;
-  double* res = nullptr;  // true residual per call
-  gf::PcgState* st = nullptr;
-  int nblocks = 0;
-  // captured chunk of iterations, keyed by preconditioner use
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
-  int graph_chunk = 0;
-};
 
 namespace gf {
 
@@ -34,13 +24,39 @@ int set_error(gf_ctx* ctx, int code, const std::string& msg);
 
 namespace {
 
-constexpr int kMaxBlocks = 2048;
 constexpr int kChunk = 32;  // iterations between convergence checks (even)
 
 double now_ms() {
   using namespace std::chrono;
   return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
+
+void drop_graphs(gf_solve_plan* P) {
+  for (auto& g : P->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+hipError_t ensure_graph(gf_ctx* ctx, gf_ell* m, const PcgBuffers& B, int key) {
+  gf_solve_plan* P = m->plan;
+  if (P->graph[key]) return hipSuccess;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  for (int it = 0; it < kChunk; ++it) {
+    e = launch_pcg_iteration(ctx->stream, B, it & 1);
+    if (e != hipSuccess) break;
+  }
+  hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+  if (e == hipSuccess) e = e2;
+  if (e == hipSuccess) e = hipGraphInstantiate(&P->graph[key], g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  return e;
+}
+
+}  // namespace
 
 int ensure_plan(gf_ctx* ctx, gf_ell* m) {
   if (m->plan && m->plan->n == m->n) return GF_OK;
@@ -52,18 +68,33 @@ int ensure_plan(gf_ctx* ctx, gf_ell* m) {
   double** vecs[] = {&P->r, &P->z, &P->p, &P->q, &P->wb, &P->wx};
   for (double** v : vecs)
     if (e == hipSuccess) e = hipMalloc(v, vb);
-  if (e == hipSuccess) e = hipMalloc(&P->part1, kMaxBlocks * sizeof(double));
-  if (e == hipSuccess) e = hipMalloc(&P->part2, 3 * kMaxBlocks * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&P->part1, kMaxPcgBlocks * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&P->part2, 3 * kMaxPcgBlocks * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&P->res, 64 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&P->st), pcg_state_bytes());
+  if (e == hipSuccess) e = hipMalloc(&P->barrier, grid_barrier_bytes());
   m->plan = P;
   if (e != hipSuccess) {
     release_solve_plan(m);
     return set_error(ctx, GF_ERR_NOMEM, std::string("gf_pcg: workspace: ") + hipGetErrorString(e));
   }
-  int64_t nb = (m->n + 255) / 256;
-  P->nblocks = int(std::max<int64_t>(1, std::min<int64_t>(nb, kMaxBlocks)));
+  P->num_cu = ctx->num_cu;
   return GF_OK;
+}
+
+// Fixes the variant and the PCG grid (the same grid for every schedule, so
+// all schedules give bitwise-identical results).
+void set_variant(gf_ell* m, int variant) {
+  gf_solve_plan* P = m->plan;
+  if (P->variant == variant && P->nblocks > 0) return;
+  drop_graphs(P);
+  P->variant = variant;
+  int nb = spmv_blocks(m->n);
+  const int pc = persistent_blocks(variant, P->num_cu);
+  P->persistent_ok = pc > 0;
+  if (pc > 0 && nb > pc) nb = pc;
+  if (nb >= 8) nb &= ~7;
+  P->nblocks = std::max(1, nb);
 }
 
 PcgBuffers make_buffers(gf_ell* m, const double* dinv) {
@@ -71,6 +102,7 @@ PcgBuffers make_buffers(gf_ell* m, const double* dinv) {
   PcgBuffers B;
   B.n = m->n;
   B.width = m->width;
+  B.variant = P->variant;
   B.val = m->d_val;
   B.col = m->d_col;
   B.rownnz = m->d_rownnz;
@@ -88,35 +120,87 @@ PcgBuffers make_buffers(gf_ell* m, const double* dinv) {
   return B;
 }
 
-hipError_t ensure_graph(gf_ctx* ctx, gf_ell* m, const PcgBuffers& B, int key) {
+int tune_spmv(gf_ctx* ctx, gf_ell* m, int force, double* ms_out, int* chosen) {
+  int st = ensure_plan(ctx, m);
+  if (st) return st;
   gf_solve_plan* P = m->plan;
-  if (P->graph[key]) return hipSuccess;
-  hipGraph_t g = nullptr;
-  hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
-  if (e != hipSuccess) return e;
-  for (int it = 0; it < kChunk; ++it) {
-    e = launch_pcg_iteration(ctx->stream, B, it & 1);
-    if (e != hipSuccess) break;
+  if (force >= 0) {
+    if (force >= kNumSpmvVariants) return set_error(ctx, GF_ERR_ARG, "gf_ell_tune: bad variant");
+    set_variant(m, force);
+    if (chosen) *chosen = force;
+    return GF_OK;
   }
-  hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
-  if (e == hipSuccess) e = e2;
-  if (e == hipSuccess) e = hipGraphInstantiate(&P->graph[key], g, nullptr, nullptr, 0);
-  if (g) (void)hipGraphDestroy(g);
-  P->graph_chunk = kChunk;
-  return e;
+  GF_HIP_CHECK(ctx, hipMemsetAsync(P->p, 0, size_t(m->n) * sizeof(double), ctx->stream));
+  const int reps = m->n > 200000 ? 10 : 30;
+  double best = 1e300;
+  int best_v = kThreadU8;
+  for (int v = 0; v < kNumSpmvVariants; ++v) {
+    set_variant(m, v);
+    PcgBuffers B = make_buffers(m, nullptr);
+    GF_HIP_CHECK(ctx, launch_spmv_pq_repeat(ctx->stream, B, 2));
+    GF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
+    GF_HIP_CHECK(ctx, launch_spmv_pq_repeat(ctx->stream, B, reps));
+    GF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
+    GF_HIP_CHECK(ctx, hipEventSynchronize(ctx->ev_stop));
+    float ms = 0.f;
+    GF_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    const double avg = double(ms) / reps;
+    if (ms_out) ms_out[v] = avg;
+    if (avg < best) {
+      best = avg;
+      best_v = v;
+    }
+  }
+  set_variant(m, best_v);
+  if (chosen) *chosen = best_v;
+  return GF_OK;
 }
 
-}  // namespace
+// Runs one right-hand side already placed in the plan's wb (and wx when
+// use_x0) to completion with the given schedule.
+int run_pcg(gf_ctx* ctx, gf_ell* m, const PcgBuffers& B, int sched, int key, int use_x0,
+            double rtol, int maxit, PcgHostState* h) {
+  gf_solve_plan* P = m->plan;
+  GF_HIP_CHECK(ctx, launch_pcg_setup(ctx->stream, B, use_x0, rtol, maxit));
+  if (sched == GF_SCHED_PERSISTENT) {
+    GF_HIP_CHECK(ctx, launch_pcg_persistent(ctx->stream, B, P->barrier));
+    GF_HIP_CHECK(ctx, read_pcg_state(ctx->stream, B, h));
+    if (h->status == GF_ERR_HIP)
+      return set_error(ctx, GF_ERR_HIP, "gf_pcg: grid barrier timed out in the persistent kernel");
+    return GF_OK;
+  }
+  GF_HIP_CHECK(ctx, read_pcg_state(ctx->stream, B, h));
+  while (!h->done) {
+    if (sched == GF_SCHED_GRAPH) {
+      GF_HIP_CHECK(ctx, ensure_graph(ctx, m, B, B.dinv ? 1 : 0));
+      GF_HIP_CHECK(ctx, hipGraphLaunch(P->graph[key], ctx->stream));
+    } else {
+      for (int it = 0; it < kChunk; ++it) GF_HIP_CHECK(ctx, launch_pcg_iteration(ctx->stream, B, it & 1));
+    }
+    GF_HIP_CHECK(ctx, read_pcg_state(ctx->stream, B, h));
+  }
+  return GF_OK;
+}
+
+int prepare_jacobi(gf_ctx* ctx, gf_ell* m) {
+  if (!m->d_dinv) GF_HIP_CHECK(ctx, hipMalloc(&m->d_dinv, size_t(m->n) * sizeof(double)));
+  if (!m->dinv_valid) {
+    GF_HIP_CHECK(ctx, launch_jacobi_inv(ctx->stream, m->n, m->d_val, m->d_diag, m->d_dinv));
+    m->dinv_valid = true;
+  }
+  return GF_OK;
+}
 
 void release_solve_plan(gf_ell* m) {
   gf_solve_plan* P = m->plan;
   if (!P) return;
-  for (auto& g : P->graph)
-    if (g) (void)hipGraphExecDestroy(g);
+  drop_graphs(P);
   double* vecs[] = {P->r, P->z, P->p, P->q, P->wb, P->wx, P->part1, P->part2, P->res};
   for (double* v : vecs)
     if (v) (void)hipFree(v);
   if (P->st) (void)hipFree(P->st);
+  if (P->barrier) (void)hipFree(P->barrier);
+  release_mixed(P);
   delete P;
   m->plan = nullptr;
 }
@@ -139,6 +223,17 @@ void gf_solve_opts_default(gf_solve_opts* o) {
   o->inner_rtol = 1e-4;
 }
 
+int gf_ell_tune(gf_ctx* ctx, gf_ell* m, int32_t force_variant, double* ms_per_variant,
+                int32_t* chosen) {
+  if (!ctx || !m) return set_error(ctx, GF_ERR_ARG, "gf_ell_tune: null argument");
+  GF_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  int c = -1;
+  int st = tune_spmv(ctx, m, force_variant, ms_per_variant, &c);
+  if (st) return st;
+  if (chosen) *chosen = c;
+  return GF_OK;
+}
+
 int gf_pcg(gf_ctx* ctx, gf_ell* m, const double* b, double* x, int32_t nrhs,
            const gf_solve_opts* opts_in, gf_solve_stats* stats) {
   if (!ctx || !m || !b || !x || nrhs < 1)
@@ -150,79 +245,71 @@ int gf_pcg(gf_ctx* ctx, gf_ell* m, const double* b, double* x, int32_t nrhs,
     return set_error(ctx, GF_ERR_ARG, "gf_pcg: rtol must be >= 0 and maxit >= 0");
   if (opts.precond != GF_PRECOND_NONE && opts.precond != GF_PRECOND_JACOBI)
     return set_error(ctx, GF_ERR_ARG, "gf_pcg: unknown preconditioner");
-  if (opts.precision != GF_PREC_FP64)
-    return set_error(ctx, GF_ERR_UNSUPPORTED, "gf_pcg: precision mode not available");
+  if (opts.precision != GF_PREC_FP64 && opts.precision != GF_PREC_MIXED)
+    return set_error(ctx, GF_ERR_ARG, "gf_pcg: unknown precision mode");
+  if (opts.schedule < GF_SCHED_AUTO || opts.schedule > GF_SCHED_PERSISTENT)
+    return set_error(ctx, GF_ERR_ARG, "gf_pcg: unknown schedule");
   if (nrhs > 64) return set_error(ctx, GF_ERR_UNSUPPORTED, "gf_pcg: at most 64 right-hand sides");
   GF_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   const double h0 = now_ms();
   int st = ensure_plan(ctx, m);
   if (st) return st;
   gf_solve_plan* P = m->plan;
+  if (P->variant < 0 && (st = tune_spmv(ctx, m, -1, nullptr, nullptr))) return st;
   const int64_t n = m->n;
   const size_t vb = size_t(n) * sizeof(double);
-
-  const double* dinv = nullptr;
-  if (opts.precond == GF_PRECOND_JACOBI) {
-    if (!m->d_dinv) GF_HIP_CHECK(ctx, hipMalloc(&m->d_dinv, vb));
-    if (!m->dinv_valid) {
-      GF_HIP_CHECK(ctx, launch_jacobi_inv(ctx->stream, n, m->d_val, m->d_diag, m->d_dinv));
-      m->dinv_valid = true;
-    }
-    dinv = m->d_dinv;
+  if (opts.precond == GF_PRECOND_JACOBI || opts.precision == GF_PREC_MIXED) {
+    st = prepare_jacobi(ctx, m);
+    if (st) return st;
   }
+  const double* dinv = opts.precond == GF_PRECOND_JACOBI ? m->d_dinv : nullptr;
   const int key = dinv ? 1 : 0;
   PcgBuffers B = make_buffers(m, dinv);
   int sched = opts.schedule == GF_SCHED_AUTO ? GF_SCHED_GRAPH : opts.schedule;
-  if (sched == GF_SCHED_PERSISTENT) sched = GF_SCHED_GRAPH;  // not built yet: same numerics
-  if (sched == GF_SCHED_GRAPH) GF_HIP_CHECK(ctx, ensure_graph(ctx, m, B, key));
+  if (sched == GF_SCHED_PERSISTENT && !P->persistent_ok)
+    return set_error(ctx, GF_ERR_UNSUPPORTED, "gf_pcg: persistent kernel cannot be co-resident");
 
   // right-hand sides and solutions resident on the device
   const double* Bd = b;
   double* Xd = x;
-  double* tmp = nullptr;
+  struct DevTmp {
+    double* p = nullptr;
+    ~DevTmp() {
+      if (p) (void)hipFree(p);
+    }
+  } tmp;
   if (!opts.device_ptrs) {
-    GF_HIP_CHECK(ctx, hipMalloc(&tmp, 2 * vb * size_t(nrhs)));
-    GF_HIP_CHECK(ctx, hipMemcpyAsync(tmp, b, vb * nrhs, hipMemcpyHostToDevice, ctx->stream));
+    GF_HIP_CHECK(ctx, hipMalloc(&tmp.p, 2 * vb * size_t(nrhs)));
+    GF_HIP_CHECK(ctx, hipMemcpyAsync(tmp.p, b, vb * nrhs, hipMemcpyHostToDevice, ctx->stream));
     if (opts.use_x0)
-      GF_HIP_CHECK(ctx, hipMemcpyAsync(tmp + n * nrhs, x, vb * nrhs, hipMemcpyHostToDevice, ctx->stream));
-    Bd = tmp;
-    Xd = tmp + n * nrhs;
+      GF_HIP_CHECK(ctx, hipMemcpyAsync(tmp.p + n * nrhs, x, vb * nrhs, hipMemcpyHostToDevice, ctx->stream));
+    Bd = tmp.p;
+    Xd = tmp.p + n * nrhs;
   }
   GF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
   std::vector<PcgHostState> hs(static_cast<size_t>(nrhs));
-  hipError_t e = hipSuccess;
-  for (int k = 0; k < nrhs && e == hipSuccess; ++k) {
-    e = hipMemcpyAsync(P->wb, Bd + k * n, vb, hipMemcpyDeviceToDevice, ctx->stream);
-    if (e == hipSuccess && opts.use_x0)
-      e = hipMemcpyAsync(P->wx, Xd + k * n, vb, hipMemcpyDeviceToDevice, ctx->stream);
-    if (e == hipSuccess) e = launch_pcg_setup(ctx->stream, B, opts.use_x0, opts.rtol, opts.maxit);
+  std::vector<int> refinements(static_cast<size_t>(nrhs), 0);
+  for (int k = 0; k < nrhs; ++k) {
+    GF_HIP_CHECK(ctx, hipMemcpyAsync(P->wb, Bd + k * n, vb, hipMemcpyDeviceToDevice, ctx->stream));
+    if (opts.use_x0)
+      GF_HIP_CHECK(ctx, hipMemcpyAsync(P->wx, Xd + k * n, vb, hipMemcpyDeviceToDevice, ctx->stream));
     PcgHostState h{};
-    if (e == hipSuccess) e = read_pcg_state(ctx->stream, B, &h);
-    while (e == hipSuccess && !h.done) {
-      if (sched == GF_SCHED_GRAPH) {
-        e = hipGraphLaunch(P->graph[key], ctx->stream);
-      } else {
-        for (int it = 0; it < kChunk && e == hipSuccess; ++it)
-          e = launch_pcg_iteration(ctx->stream, B, it & 1);
-      }
-      if (e == hipSuccess) e = read_pcg_state(ctx->stream, B, &h);
-    }
-    if (e == hipSuccess) e = launch_true_residual(ctx->stream, B, P->res + k);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(Xd + k * n, P->wx, vb, hipMemcpyDeviceToDevice, ctx->stream);
+    if (opts.precision == GF_PREC_MIXED)
+      st = run_mixed(ctx, m, opts, &h, &refinements[size_t(k)]);
+    else
+      st = run_pcg(ctx, m, B, sched, key, opts.use_x0, opts.rtol, opts.maxit, &h);
+    if (st) return st;
+    GF_HIP_CHECK(ctx, launch_true_residual(ctx->stream, B, P->res + k));
+    GF_HIP_CHECK(ctx, hipMemcpyAsync(Xd + k * n, P->wx, vb, hipMemcpyDeviceToDevice, ctx->stream));
     hs[size_t(k)] = h;
   }
-  if (e == hipSuccess) e = hipEventRecord(ctx->ev_stop, ctx->stream);
-  if (e == hipSuccess) e = hipEventSynchronize(ctx->ev_stop);
+  GF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
+  GF_HIP_CHECK(ctx, hipEventSynchronize(ctx->ev_stop));
   float ms = 0.f;
-  if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
+  GF_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
   std::vector<double> res(size_t(nrhs), 0.0);
-  if (e == hipSuccess)
-    e = hipMemcpy(res.data(), P->res, size_t(nrhs) * sizeof(double), hipMemcpyDeviceToHost);
-  if (e == hipSuccess && !opts.device_ptrs)
-    e = hipMemcpy(x, Xd, vb * nrhs, hipMemcpyDeviceToHost);
-  if (tmp) (void)hipFree(tmp);
-  if (e != hipSuccess) return set_error(ctx, GF_ERR_HIP, std::string("gf_pcg: ") + hipGetErrorString(e));
+  GF_HIP_CHECK(ctx, hipMemcpy(res.data(), P->res, size_t(nrhs) * sizeof(double), hipMemcpyDeviceToHost));
+  if (!opts.device_ptrs) GF_HIP_CHECK(ctx, hipMemcpy(x, Xd, vb * nrhs, hipMemcpyDeviceToHost));
 
   gf_solve_stats S{};
   S.converged = 1;
@@ -230,6 +317,7 @@ int gf_pcg(gf_ctx* ctx, gf_ell* m, const double* b, double* x, int32_t nrhs,
   for (int k = 0; k < nrhs; ++k) {
     const PcgHostState& h = hs[size_t(k)];
     S.iterations = std::max(S.iterations, h.it);
+    S.refinements = std::max(S.refinements, refinements[size_t(k)]);
     const double rel = h.bnorm2 > 0 ? std::sqrt(h.rr / h.bnorm2) : std::sqrt(h.rr);
     S.relres = std::max(S.relres, rel);
     S.true_relres = std::max(S.true_relres, res[size_t(k)]);
@@ -257,10 +345,10 @@ int gf_profile_spmv(gf_ctx* ctx, gf_ell* m, int32_t nrep, double* ms_avg) {
   GF_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   int st = ensure_plan(ctx, m);
   if (st) return st;
+  if (m->plan->variant < 0 && (st = tune_spmv(ctx, m, -1, nullptr, nullptr))) return st;
   PcgBuffers B = make_buffers(m, nullptr);
-  // a fixed, finite direction vector
   GF_HIP_CHECK(ctx, hipMemsetAsync(m->plan->p, 0, size_t(m->n) * sizeof(double), ctx->stream));
-  GF_HIP_CHECK(ctx, launch_spmv_pq_repeat(ctx->stream, B, 1));  // warm caches / code
+  GF_HIP_CHECK(ctx, launch_spmv_pq_repeat(ctx->stream, B, 1));
   GF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
   GF_HIP_CHECK(ctx, launch_spmv_pq_repeat(ctx->stream, B, nrep));
   GF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));

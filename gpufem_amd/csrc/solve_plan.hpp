@@ -1,8 +1,36 @@
-// Solver workspace cached on an ELL matrix (see solve.cpp).
+// Solver workspace cached on an ELL matrix (solve.cpp, mixed.cpp).
 #pragma once
 
+#include <string>
+
 #include "gf_internal.hpp"
+#include "launch.hpp"
+
+struct gf_mixed_ws;  // mixed-precision workspace (mixed.cpp)
+
+struct gf_solve_plan {
+  int64_t n = 0;
+  int num_cu = 0;
+  int variant = -1;  // SpMV variant (-1: not tuned yet)
+  int nblocks = 0;   // PCG workgroups
+  bool persistent_ok = false;
+  double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  double *wb = nullptr, *wx = nullptr;  // right-hand side and solution being solved
+  double *part1 = nullptr, *part2 = nullptr;
+  double* res = nullptr;  // true residual per right-hand side
+  gf::PcgState* st = nullptr;
+  void* barrier = nullptr;  // grid barrier of the persistent kernel
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // keyed by preconditioner use
+  gf_mixed_ws* mixed = nullptr;
+};
 
 namespace gf {
 void release_solve_plan(gf_ell* m);
+int ensure_plan(gf_ctx* ctx, gf_ell* m);
+void set_variant(gf_ell* m, int variant);
+int tune_spmv(gf_ctx* ctx, gf_ell* m, int force, double* ms_out, int* chosen);
+PcgBuffers make_buffers(gf_ell* m, const double* dinv);
+int prepare_jacobi(gf_ctx* ctx, gf_ell* m);
+int run_pcg(gf_ctx* ctx, gf_ell* m, const PcgBuffers& B, int sched, int key, int use_x0,
+            double rtol, int maxit, PcgHostState* h);
 }  // namespace gf

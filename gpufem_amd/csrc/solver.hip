@@ -1,17 +1,26 @@
-// ELL SpMV and the Jacobi-preconditioned CG, FP64.
+// ELL SpMV and the Jacobi-preconditioned CG in FP64.
 //
-// The CG loop runs without host round trips: every scalar (alpha, beta,
-// rho, ||r||^2) lives in a device PcgState, and each kernel reduces the
-// per-block partials of the previous one in a fixed order, so results are
-// bitwise reproducible for a given problem size.  One iteration is three
-// launches:
-//   k_spmv_pq   q = K p, partial p.q
-//   k_update    alpha = rho / p.q; x += alpha p; r -= alpha q; z = M^-1 r;
-//               partials r.z, r.r
-//   k_direction beta = r.z / rho; p = z + beta p; block 0 advances the state
-// Kernels return immediately once the state says the solve is done, so the
-// host can enqueue (or replay as a hipGraph) fixed chunks of iterations and
-// check convergence between chunks.
+// Rows are split into contiguous blocks, one per workgroup, and workgroup
+// ids are remapped so that each XCD (workgroups are dispatched round-robin
+// over the 8 XCDs) owns a contiguous 1/8 of the rows: the x-gathers of
+// neighbouring rows then hit the same L2.
+//
+// SpMV variants (chosen per matrix by the tuner in solve.cpp):
+//   kThreadU1/U4/U8  one thread per row, slots streamed 1/4/8 at a time
+//                    (more loads in flight per thread); same summation
+//                    order as the reference's Axy, so all three are bitwise
+//                    identical.
+//   kTile2/kTile4    2 or 4 waves share a 64-row tile, each taking every
+//                    2nd/4th slot; partial sums combined through LDS.
+//
+// CG schedules: three launches per iteration (k_spmv_pq, k_update,
+// k_direction) with all scalars in a device PcgState, so the host can
+// enqueue or graph-replay chunks of iterations; or one cooperative
+// persistent kernel (k_pcg_persistent) with three grid barriers per
+// iteration.  Every block reduces the per-block partials in the same fixed
+// order, so all blocks take the same convergence decision and results are
+// bitwise reproducible for a given matrix and variant.
+#include "device_common.hpp"
 #include "gf_internal.hpp"
 #include "launch.hpp"
 
@@ -29,50 +38,27 @@ struct PcgState {
   int32_t maxit;
 };
 
+struct GridBarrier {
+  unsigned count;
+  unsigned gen;
+  int abort;
+  int pad;
+};
+
 namespace {
 
-constexpr int kNT = 256;  // threads per block for the vector kernels
+using namespace dev;
 
-// Sum over the block; every thread gets the same (bitwise) result.
-__device__ inline double block_sum(double v, double* sh) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double t = 0.0;
-#pragma unroll
-  for (int w = 0; w < kNT / 64; ++w) t += sh[w];
-  return t;
-}
-
-// Sum of np partials, identical in every block.
-__device__ inline double reduce_partials(const double* part, int np, double* sh) {
-  double v = 0.0;
-  for (int i = threadIdx.x; i < np; i += kNT) v += part[i];
-  return block_sum(v, sh);
-}
-
-__device__ inline double ell_row(int64_t i, int64_t n, const double* __restrict__ val,
-                                 const int32_t* __restrict__ col, int32_t rn,
-                                 const double* __restrict__ x) {
-  double acc = 0.0;
-  for (int32_t s = 0; s < rn; ++s) {
-    const int64_t idx = i + int64_t(s) * n;
-    acc += __builtin_nontemporal_load(val + idx) * x[__builtin_nontemporal_load(col + idx)];
-  }
-  return acc;
-}
-
+// ---------------------------------------------------------------------------
+// standalone SpMV (multi-RHS by repeated sweeps)
+// ---------------------------------------------------------------------------
+template <int V>
 __global__ __launch_bounds__(kNT) void k_spmv(int64_t n, const double* __restrict__ val,
                                               const int32_t* __restrict__ col,
                                               const int32_t* __restrict__ rownnz,
                                               const double* __restrict__ x,
-                                              double* __restrict__ y, int nrhs) {
-  for (int64_t i = int64_t(blockIdx.x) * kNT + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * kNT) {
-    const int32_t rn = rownnz[i];
-    for (int k = 0; k < nrhs; ++k) y[int64_t(k) * n + i] = ell_row(i, n, val, col, rn, x + int64_t(k) * n);
-  }
+                                              double* __restrict__ y) {
+  for_rows_spmv<V, double>(n, val, col, rownnz, x, [&](int64_t i, double v) { y[i] = v; });
 }
 
 __global__ void k_jacobi_inv(int64_t n, const double* val, const int32_t* diag, double* dinv) {
@@ -84,12 +70,50 @@ __global__ void k_jacobi_inv(int64_t n, const double* val, const int32_t* diag, 
   }
 }
 
+// ---------------------------------------------------------------------------
+// PCG phases (shared by the multi-launch and the persistent schedules)
+// ---------------------------------------------------------------------------
+template <int V>
+__device__ inline double phase_spmv_pq(const PcgBuffers& B) {
+  double pq = 0.0;
+  for_rows_spmv<V, double>(B.n, B.val, B.col, B.rownnz, B.p, [&](int64_t i, double qi) {
+    B.q[i] = qi;
+    pq += B.p[i] * qi;
+  });
+  return pq;
+}
+
+__device__ inline void phase_update(const PcgBuffers& B, double alpha, double* rz_out,
+                                    double* rr_out) {
+  int64_t r0, r1;
+  row_range(B.n, &r0, &r1);
+  double rz = 0, rr = 0;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += kNT) {
+    B.x[i] += alpha * B.p[i];
+    const double ri = B.r[i] - alpha * B.q[i];
+    const double zi = B.dinv ? B.dinv[i] * ri : ri;
+    B.r[i] = ri;
+    B.z[i] = zi;
+    rz += ri * zi;
+    rr += ri * ri;
+  }
+  *rz_out = rz;
+  *rr_out = rr;
+}
+
+__device__ inline void phase_direction(const PcgBuffers& B, double beta) {
+  int64_t r0, r1;
+  row_range(B.n, &r0, &r1);
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += kNT) B.p[i] = B.z[i] + beta * B.p[i];
+}
+
 // r = b - q (use_x0) or x = 0, r = b; z = M^-1 r; p = z; partials r.z, r.r, b.b
 __global__ __launch_bounds__(kNT) void k_pcg_init(PcgBuffers B, int use_x0) {
   __shared__ double sh[kNT / 64];
+  int64_t r0, r1;
+  row_range(B.n, &r0, &r1);
   double rz = 0, rr = 0, bb = 0;
-  for (int64_t i = int64_t(blockIdx.x) * kNT + threadIdx.x; i < B.n;
-       i += int64_t(gridDim.x) * kNT) {
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += kNT) {
     const double bi = B.b[i];
     double ri;
     if (use_x0) {
@@ -136,17 +160,11 @@ __global__ __launch_bounds__(kNT) void k_pcg_init_finish(PcgBuffers B, int nb, d
   }
 }
 
+template <int V>
 __global__ __launch_bounds__(kNT) void k_spmv_pq(PcgBuffers B) {
   if (B.st->done) return;
   __shared__ double sh[kNT / 64];
-  double pq = 0.0;
-  for (int64_t i = int64_t(blockIdx.x) * kNT + threadIdx.x; i < B.n;
-       i += int64_t(gridDim.x) * kNT) {
-    const double qi = ell_row(i, B.n, B.val, B.col, B.rownnz[i], B.p);
-    B.q[i] = qi;
-    pq += B.p[i] * qi;
-  }
-  pq = block_sum(pq, sh);
+  const double pq = block_sum(phase_spmv_pq<V>(B), sh);
   if (threadIdx.x == 0) B.part1[blockIdx.x] = pq;
 }
 
@@ -162,18 +180,8 @@ __global__ __launch_bounds__(kNT) void k_update(PcgBuffers B, int parity) {
     }
     return;
   }
-  const double alpha = st->rho[parity] / pq;
-  double rz = 0, rr = 0;
-  for (int64_t i = int64_t(blockIdx.x) * kNT + threadIdx.x; i < B.n;
-       i += int64_t(gridDim.x) * kNT) {
-    B.x[i] += alpha * B.p[i];
-    const double ri = B.r[i] - alpha * B.q[i];
-    const double zi = B.dinv ? B.dinv[i] * ri : ri;
-    B.r[i] = ri;
-    B.z[i] = zi;
-    rz += ri * zi;
-    rr += ri * ri;
-  }
+  double rz, rr;
+  phase_update(B, st->rho[parity] / pq, &rz, &rr);
   rz = block_sum(rz, sh);
   rr = block_sum(rr, sh);
   if (threadIdx.x == 0) {
@@ -189,10 +197,7 @@ __global__ __launch_bounds__(kNT) void k_direction(PcgBuffers B, int parity) {
   const int nb = gridDim.x;
   const double rz = reduce_partials(B.part2, nb, sh);
   const double rr = reduce_partials(B.part2 + nb, nb, sh);
-  const double beta = rz / st->rho[parity];
-  for (int64_t i = int64_t(blockIdx.x) * kNT + threadIdx.x; i < B.n;
-       i += int64_t(gridDim.x) * kNT)
-    B.p[i] = B.z[i] + beta * B.p[i];
+  phase_direction(B, rz / st->rho[parity]);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->rho[parity ^ 1] = rz;
     st->rr = rr;
@@ -202,12 +207,116 @@ __global__ __launch_bounds__(kNT) void k_direction(PcgBuffers B, int parity) {
   }
 }
 
-// partial sums of (b - q)^2 then ||b - q|| / ||b||
+// ---------------------------------------------------------------------------
+// bounded grid barrier for the cooperative kernel
+// ---------------------------------------------------------------------------
+constexpr long kSpinLimit = 1L << 24;  // ~1 s of polling: a barrier never takes that long
+
+__device__ inline bool grid_barrier(GridBarrier* bar, int* s_flag) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned nb = gridDim.x;
+    const unsigned g = __hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned arrived =
+        __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (arrived == nb - 1) {
+      __hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&bar->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      long spins = 0;
+      while (__hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        __builtin_amdgcn_s_sleep(2);
+        if (__hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        if (++spins > kSpinLimit) {
+          __hip_atomic_store(&bar->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+      }
+    }
+    *s_flag = __hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return *s_flag == 0;
+}
+
+// All iterations in one launch.  The grid never exceeds the co-resident
+// block count (cooperative launch), every block runs the same number of
+// barriers, and the convergence test uses values every block computed
+// identically, so all blocks leave the loop together.
+template <int V>
+__global__ __launch_bounds__(kNT) void k_pcg_persistent(PcgBuffers B, GridBarrier* bar) {
+  __shared__ double sh[kNT / 64];
+  __shared__ int s_flag;
+  PcgState* st = B.st;
+  if (st->done) return;  // read before any block can change it (it only changes at exit)
+  if (threadIdx.x == 0) s_flag = 0;
+  __syncthreads();
+  int it = st->it;
+  double rho = st->rho[it & 1];
+  const double rtol2 = st->rtol2, bnorm2 = st->bnorm2;
+  const int maxit = st->maxit;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  for (;;) {
+    const double pq_part = block_sum(phase_spmv_pq<V>(B), sh);
+    if (threadIdx.x == 0) B.part1[blockIdx.x] = pq_part;
+    if (!grid_barrier(bar, &s_flag)) break;
+    const double pq = reduce_partials(B.part1, gridDim.x, sh);
+    if (!(pq > 0.0)) {
+      if (lead) {
+        st->status = GF_ERR_BREAKDOWN;
+        st->done = 1;
+      }
+      break;
+    }
+    double rz, rr;
+    phase_update(B, rho / pq, &rz, &rr);
+    rz = block_sum(rz, sh);
+    rr = block_sum(rr, sh);
+    if (threadIdx.x == 0) {
+      B.part2[blockIdx.x] = rz;
+      B.part2[gridDim.x + blockIdx.x] = rr;
+    }
+    if (!grid_barrier(bar, &s_flag)) break;
+    rz = reduce_partials(B.part2, gridDim.x, sh);
+    rr = reduce_partials(B.part2 + gridDim.x, gridDim.x, sh);
+    phase_direction(B, rz / rho);
+    rho = rz;
+    ++it;
+    const bool finished = rr <= rtol2 * bnorm2 || it >= maxit;
+    if (lead) {
+      st->rho[it & 1] = rz;
+      st->rr = rr;
+      st->it = it;
+      if (finished) st->done = 1;
+    }
+    if (finished) break;
+    if (!grid_barrier(bar, &s_flag)) break;
+  }
+  if (lead && s_flag) {
+    st->status = GF_ERR_HIP;
+    st->done = 1;
+  }
+}
+
+__global__ void k_barrier_reset(GridBarrier* bar) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    bar->count = 0;
+    bar->gen = 0;
+    bar->abort = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// residual and helpers
+// ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kNT) void k_resid_partial(PcgBuffers B) {
   __shared__ double sh[kNT / 64];
+  int64_t r0, r1;
+  row_range(B.n, &r0, &r1);
   double s = 0.0;
-  for (int64_t i = int64_t(blockIdx.x) * kNT + threadIdx.x; i < B.n;
-       i += int64_t(gridDim.x) * kNT) {
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += kNT) {
     const double d = B.b[i] - B.q[i];
     s += d * d;
   }
@@ -224,6 +333,13 @@ __global__ __launch_bounds__(kNT) void k_resid_finish(PcgBuffers B, int nb, doub
   }
 }
 
+__global__ void k_state_live(PcgState* st) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    st->done = 0;
+    st->status = GF_OK;
+  }
+}
+
 inline int grid_for(int64_t work, int threads, int cap) {
   int64_t b = (work + threads - 1) / threads;
   if (b < 1) b = 1;
@@ -234,14 +350,30 @@ inline int grid_for(int64_t work, int threads, int cap) {
 }  // namespace
 
 size_t pcg_state_bytes() { return sizeof(PcgState); }
+size_t grid_barrier_bytes() { return sizeof(GridBarrier); }
 
-hipError_t launch_spmv(hipStream_t s, int64_t n, int32_t width, const double* val,
-                       const int32_t* col, const int32_t* rownnz, const double* x,
-                       double* y, int nrhs) {
-  (void)width;
-  hipLaunchKernelGGL(k_spmv, dim3(grid_for(n, kNT, 8192)), dim3(kNT), 0, s, n, val, col,
-                     rownnz, x, y, nrhs);
-  return hipGetLastError();
+int spmv_blocks(int64_t n) {
+  // enough rows per block to amortize the partition, at most kMaxPcgBlocks;
+  // a multiple of 8 so the XCD remap applies
+  int64_t nb = (n + 1023) / 1024;
+  if (nb > kMaxPcgBlocks) nb = kMaxPcgBlocks;
+  if (nb >= 8) nb &= ~int64_t(7);
+  if (nb < 1) nb = 1;
+  return int(nb);
+}
+
+hipError_t launch_spmv(hipStream_t s, int variant, int64_t n, const double* val,
+                       const int32_t* col, const int32_t* rownnz, const double* x, double* y,
+                       int nrhs) {
+  const dim3 g(spmv_blocks(n));
+  for (int k = 0; k < nrhs; ++k) {
+    const double* xk = x + int64_t(k) * n;
+    double* yk = y + int64_t(k) * n;
+    GF_DISPATCH_VARIANT(variant, k_spmv, g, s, n, val, col, rownnz, xk, yk);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_jacobi_inv(hipStream_t s, int64_t n, const double* val,
@@ -255,7 +387,7 @@ hipError_t launch_pcg_setup(hipStream_t s, const PcgBuffers& B, int use_x0, doub
                             int maxit) {
   hipError_t e;
   if (use_x0) {
-    e = launch_spmv(s, B.n, B.width, B.val, B.col, B.rownnz, B.x, B.q, 1);
+    e = launch_spmv(s, B.variant, B.n, B.val, B.col, B.rownnz, B.x, B.q, 1);
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_pcg_init, dim3(B.nblocks), dim3(kNT), 0, s, B, use_x0);
@@ -267,7 +399,7 @@ hipError_t launch_pcg_setup(hipStream_t s, const PcgBuffers& B, int use_x0, doub
 
 hipError_t launch_pcg_iteration(hipStream_t s, const PcgBuffers& B, int parity) {
   hipError_t e;
-  hipLaunchKernelGGL(k_spmv_pq, dim3(B.nblocks), dim3(kNT), 0, s, B);
+  GF_DISPATCH_VARIANT(B.variant, k_spmv_pq, dim3(B.nblocks), s, B);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_update, dim3(B.nblocks), dim3(kNT), 0, s, B, parity);
   if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -275,21 +407,40 @@ hipError_t launch_pcg_iteration(hipStream_t s, const PcgBuffers& B, int parity) 
   return hipGetLastError();
 }
 
-__global__ void k_state_live(PcgState* st) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    st->done = 0;
-    st->status = GF_OK;
+int persistent_blocks(int variant, int num_cu) {
+  int per_cu = 0;
+  hipError_t e = hipErrorInvalidValue;
+  switch (variant) {
+    case kThreadU1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_persistent<kThreadU1>, kNT, 0); break;
+    case kThreadU4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_persistent<kThreadU4>, kNT, 0); break;
+    case kThreadU8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_persistent<kThreadU8>, kNT, 0); break;
+    case kTile2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_persistent<kTile2>, kNT, 0); break;
+    case kTile4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_persistent<kTile4>, kNT, 0); break;
+    default: break;
   }
+  if (e != hipSuccess || per_cu < 1) return 0;
+  int64_t nb = int64_t(per_cu) * num_cu;
+  if (nb > kMaxPcgBlocks) nb = kMaxPcgBlocks;
+  nb &= ~int64_t(7);
+  return int(nb);
 }
 
-hipError_t launch_spmv_pq_repeat(hipStream_t s, const PcgBuffers& B, int nrep) {
-  hipLaunchKernelGGL(k_state_live, dim3(1), dim3(64), 0, s, B.st);
+hipError_t launch_pcg_persistent(hipStream_t s, const PcgBuffers& B, void* barrier) {
+  GridBarrier* bar = static_cast<GridBarrier*>(barrier);
+  hipLaunchKernelGGL(k_barrier_reset, dim3(1), dim3(64), 0, s, bar);
   hipError_t e = hipGetLastError();
-  for (int i = 0; i < nrep && e == hipSuccess; ++i) {
-    hipLaunchKernelGGL(k_spmv_pq, dim3(B.nblocks), dim3(kNT), 0, s, B);
-    e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  PcgBuffers Bc = B;
+  void* args[] = {&Bc, &bar};
+  const dim3 g(B.nblocks), blk(kNT);
+  switch (B.variant) {
+    case kThreadU1: return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent<kThreadU1>), g, blk, args, 0, s);
+    case kThreadU4: return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent<kThreadU4>), g, blk, args, 0, s);
+    case kThreadU8: return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent<kThreadU8>), g, blk, args, 0, s);
+    case kTile2: return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent<kTile2>), g, blk, args, 0, s);
+    case kTile4: return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent<kTile4>), g, blk, args, 0, s);
+    default: return hipErrorInvalidValue;
   }
-  return e;
 }
 
 hipError_t read_pcg_state(hipStream_t s, const PcgBuffers& B, PcgHostState* h) {
@@ -307,8 +458,18 @@ hipError_t read_pcg_state(hipStream_t s, const PcgBuffers& B, PcgHostState* h) {
   return hipSuccess;
 }
 
+hipError_t launch_spmv_pq_repeat(hipStream_t s, const PcgBuffers& B, int nrep) {
+  hipLaunchKernelGGL(k_state_live, dim3(1), dim3(64), 0, s, B.st);
+  hipError_t e = hipGetLastError();
+  for (int i = 0; i < nrep && e == hipSuccess; ++i) {
+    GF_DISPATCH_VARIANT(B.variant, k_spmv_pq, dim3(B.nblocks), s, B);
+    e = hipGetLastError();
+  }
+  return e;
+}
+
 hipError_t launch_true_residual(hipStream_t s, const PcgBuffers& B, double* out_dev) {
-  hipError_t e = launch_spmv(s, B.n, B.width, B.val, B.col, B.rownnz, B.x, B.q, 1);
+  hipError_t e = launch_spmv(s, B.variant, B.n, B.val, B.col, B.rownnz, B.x, B.q, 1);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_resid_partial, dim3(B.nblocks), dim3(kNT), 0, s, B);
   if ((e = hipGetLastError()) != hipSuccess) return e;

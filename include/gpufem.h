@@ -206,6 +206,18 @@ void gf_solve_opts_default(gf_solve_opts* o);
 int gf_pcg(gf_ctx* ctx, gf_ell* m, const double* b, double* x, int32_t nrhs,
            const gf_solve_opts* opts, gf_solve_stats* stats);
 
+/* SpMV schedule of this matrix (the autotuner).  Variants: 0..2 one thread
+ * per row with 1/4/8 slots in flight, 3..4 two/four wavefronts per 64-row
+ * tile.  force_variant >= 0 fixes a variant; -1 times every variant on this
+ * matrix and keeps the fastest (gf_pcg does this on first use).
+ * ms_per_variant (may be NULL) receives GF_SPMV_VARIANTS averages.  No
+ * reference counterpart; it plays the role of SetOclStrategy /
+ * SetOclLocalSize (src/SPRmatrix/SPRmatrix.h:110-121) chosen by measurement
+ * instead of by hand. */
+#define GF_SPMV_VARIANTS 5
+int gf_ell_tune(gf_ctx* ctx, gf_ell* m, int32_t force_variant, double* ms_per_variant,
+                int32_t* chosen);
+
 /* Average duration (ms) of the SpMV kernel the solver runs each iteration
  * (q = K p fused with the p.q partial sums), timed with HIP events over nrep
  * back-to-back launches on the context's stream.  No reference counterpart:

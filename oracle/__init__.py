@@ -223,8 +223,10 @@ def greedy_coloring(conn, nnodes):
 
 
 def assemble(dim, ngauss, conn, coords, E, nu, elem_E=None, skip_zero=False, nthreads=None,
-             colors=None):
-    """Colored FP64 assembly into a fresh ELL matrix; returns (EllMatrix, seconds)."""
+             colors=None, into=None):
+    """Colored FP64 assembly; returns (EllMatrix, seconds).  With `into`, the
+    contributions are added to that matrix (numeric re-assembly when its
+    values were zeroed), else a fresh matrix is made."""
     conn = np.ascontiguousarray(conn, dtype=np.int32)
     coords = np.ascontiguousarray(coords, dtype=np.float64)
     nnodes = coords.shape[0]
@@ -232,7 +234,7 @@ def assemble(dim, ngauss, conn, coords, E, nu, elem_E=None, skip_zero=False, nth
         colors, nc = greedy_coloring(conn, nnodes)
     else:
         nc = int(colors.max())
-    K = EllMatrix(nnodes * dim, skip_zero=skip_zero)
+    K = into if into is not None else EllMatrix(nnodes * dim, skip_zero=skip_zero)
     secs = C.c_double()
     eE = None if elem_E is None else np.ascontiguousarray(elem_E, dtype=np.float64)
     st = load().orc_assemble(K.m, dim, conn.shape[1], ngauss, conn.shape[0], _p(conn),

@@ -181,3 +181,83 @@ def test_pcg_error_paths(gpu_ctx):
     assert st.code == gf.GF_ERR_BREAKDOWN
     x, st = gpu_ctx.pcg(K, np.zeros(32))
     assert st.iterations == 0 and np.all(x == 0)
+
+
+# ---- SpMV variants, schedules, mixed precision ------------------------------------------
+def assembled(ctx, t, nx, ny, nz=0):
+    p = P.structured(t, nx, ny, nz)
+    mesh = ctx.upload_mesh(p.elem_type, p.ngauss, p.conn, p.coords, p.E, p.nu)
+    K = ctx.symbolic(mesh)
+    ctx.assemble(mesh, K)
+    ctx.apply_penalty(K, p.fixed_dofs)
+    return p, K
+
+
+def test_spmv_variants_match_oracle(gpu_ctx):
+    g = gf.Grid(gf.GF_BRICK20, 5, 4, 3)
+    conn, coords = g.fill()
+    Ko, _ = O.assemble(3, 3, conn, coords, 210000.0, 0.3, skip_zero=False)
+    K = upload_oracle(gpu_ctx, Ko)
+    x = np.random.default_rng(5).normal(size=g.ndof)
+    yo = Ko.axy(x)
+    ys = []
+    for v in range(5):
+        gpu_ctx.tune(K, v)
+        y, _ = gpu_ctx.spmv(K, x)
+        assert np.abs(y - yo).max() <= 1e-13 * np.abs(yo).max(), v
+        ys.append(y)
+    # one thread per row: same summation order whatever the unroll
+    assert np.array_equal(ys[0], ys[1]) and np.array_equal(ys[1], ys[2])
+    chosen, ms = gpu_ctx.tune(K)
+    assert 0 <= chosen < 5 and np.all(ms > 0)
+
+
+@pytest.mark.parametrize("variant", [0, 2, 3, 4])
+def test_persistent_matches_kernel_schedule(gpu_ctx, variant):
+    p, K = assembled(gpu_ctx, gf.GF_Q4, 64, 32)
+    gpu_ctx.tune(K, variant)
+    x1, s1 = gpu_ctx.pcg(K, p.f, schedule=gf.GF_SCHED_KERNELS)
+    x2, s2 = gpu_ctx.pcg(K, p.f, schedule=gf.GF_SCHED_PERSISTENT)
+    assert s1.iterations == s2.iterations and s2.converged
+    assert np.array_equal(x1, x2)
+
+
+def test_persistent_breakdown_and_maxit(gpu_ctx):
+    n = 4
+    val = np.array([[1.0, -1.0, 1.0, 1.0]])
+    col = np.arange(n, dtype=np.int32)[None, :]
+    Kn = gpu_ctx.upload_ell(val, col, np.ones(n, dtype=np.int32))
+    x, st = gpu_ctx.pcg(Kn, np.array([0.0, 1.0, 0.0, 0.0]), precond=gf.GF_PRECOND_NONE,
+                        schedule=gf.GF_SCHED_PERSISTENT, raise_on_fail=False)
+    assert st.code == gf.GF_ERR_BREAKDOWN
+    p, K = assembled(gpu_ctx, gf.GF_Q4, 16, 8)
+    x, st = gpu_ctx.pcg(K, p.f, maxit=7, schedule=gf.GF_SCHED_PERSISTENT, raise_on_fail=False)
+    assert st.code == gf.GF_ERR_NOCONV and st.iterations == 7
+
+
+@pytest.mark.parametrize("t,nx,ny,nz", [(gf.GF_Q4, 64, 32, 0), (gf.GF_Q8, 32, 16, 0),
+                                        (gf.GF_BRICK8, 8, 8, 8), (gf.GF_BRICK20, 4, 4, 4)])
+def test_mixed_precision_meets_fp64_tolerance(gpu_ctx, t, nx, ny, nz):
+    """FP32 inner iterations inside FP64 refinement reach the same true
+    residual as the FP64 solve; at a tight tolerance the displacements agree
+    with the FP64 oracle's to 1e-9 relative."""
+    p, K = assembled(gpu_ctx, t, nx, ny, nz)
+    xm, sm = gpu_ctx.pcg(K, p.f, precision=gf.GF_PREC_MIXED, rtol=1e-8)
+    assert sm.converged and sm.true_relres <= 1.05e-8 and sm.refinements >= 1
+    x64, s64 = gpu_ctx.pcg(K, p.f, rtol=1e-8)
+    assert sm.iterations <= 1.25 * s64.iterations + 10
+    Ko, _ = O.assemble(p.dim, p.ngauss, p.conn, p.coords, p.E, p.nu, skip_zero=False)
+    O.apply_penalty(Ko, p.dim, p.supports)
+    xo, _, _, _ = O.pcg_jacobi(Ko, p.f, rtol=1e-12)
+    xt, st = gpu_ctx.pcg(K, p.f, precision=gf.GF_PREC_MIXED, rtol=1e-12)
+    assert st.converged
+    assert np.linalg.norm(xt - xo) <= 1e-9 * np.linalg.norm(xo)
+
+
+def test_mixed_warm_start_and_zero_rhs(gpu_ctx):
+    p, K = assembled(gpu_ctx, gf.GF_Q4, 24, 12)
+    x, s = gpu_ctx.pcg(K, p.f, precision=gf.GF_PREC_MIXED)
+    xw, sw = gpu_ctx.pcg(K, p.f, x0=x, precision=gf.GF_PREC_MIXED)
+    assert sw.iterations == 0
+    xz, sz = gpu_ctx.pcg(K, np.zeros(p.ndof), precision=gf.GF_PREC_MIXED)
+    assert sz.iterations == 0 and np.all(xz == 0)
