@@ -489,7 +489,8 @@ int gf_spmv(gf_ctx* ctx, const gf_ell* m, const double* x, double* y, int32_t nr
   if (e == hipSuccess) e = hipMemcpyAsync(dx, x, vb, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipEventRecord(ctx->ev_start, ctx->stream);
   if (e == hipSuccess)
-    e = launch_spmv(ctx->stream, m->n, m->width, m->d_val, m->d_col, m->d_rownnz, dx, dy, nrhs);
+    e = launch_spmv(ctx->stream, m->plan && m->plan->variant >= 0 ? m->plan->variant : kThreadU8,
+                    m->n, m->d_val, m->d_col, m->d_rownnz, dx, dy, nrhs);
   if (e == hipSuccess) e = hipEventRecord(ctx->ev_stop, ctx->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(y, dy, vb, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
