@@ -315,22 +315,27 @@ __global__ __launch_bounds__(kNT) void k_resid_partial(PcgBuffers B) {
   __shared__ double sh[kNT / 64];
   int64_t r0, r1;
   row_range(B.n, &r0, &r1);
-  double s = 0.0;
+  double s = 0.0, bb = 0.0;
   for (int64_t i = r0 + threadIdx.x; i < r1; i += kNT) {
-    const double d = B.b[i] - B.q[i];
+    const double bi = B.b[i];
+    const double d = bi - B.q[i];
     s += d * d;
+    bb += bi * bi;
   }
   s = block_sum(s, sh);
-  if (threadIdx.x == 0) B.part1[blockIdx.x] = s;
+  bb = block_sum(bb, sh);
+  if (threadIdx.x == 0) {
+    B.part1[blockIdx.x] = s;
+    B.part2[blockIdx.x] = bb;
+  }
 }
 
+// ||b - K x|| / ||b|| (or ||b - K x|| when b = 0) from the partials above
 __global__ __launch_bounds__(kNT) void k_resid_finish(PcgBuffers B, int nb, double* out) {
   __shared__ double sh[kNT / 64];
   const double s = reduce_partials(B.part1, nb, sh);
-  if (threadIdx.x == 0) {
-    const double bb = B.st->bnorm2;
-    out[0] = bb > 0.0 ? sqrt(s / bb) : sqrt(s);
-  }
+  const double bb = reduce_partials(B.part2, nb, sh);
+  if (threadIdx.x == 0) out[0] = bb > 0.0 ? sqrt(s / bb) : sqrt(s);
 }
 
 __global__ void k_state_live(PcgState* st) {
