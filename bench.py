@@ -99,7 +99,7 @@ def main():
     ap.add_argument("--workload", default="q8_plate_1024x512")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=25.0)
-    ap.add_argument("--schedule", default="persistent",
+    ap.add_argument("--schedule", default="graph",
                     choices=["kernels", "graph", "persistent"])
     ap.add_argument("--compare", default="all", choices=["all", "none"],
                     help="also time the other FP64 schedules and the mixed solve")

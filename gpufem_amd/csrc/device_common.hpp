@@ -42,17 +42,19 @@ __device__ inline int logical_block() {
   return (b & 7) * (nb >> 3) + (b >> 3);
 }
 
-// Contiguous rows [r0, r1) of this workgroup (multiples of a wavefront).
-__device__ inline void row_range(int64_t n, int64_t* r0, int64_t* r1) {
-  const int64_t nb = gridDim.x;
-  int64_t R = (n + nb - 1) / nb;
-  R = (R + 63) & ~int64_t(63);
-  const int64_t lb = logical_block();
-  int64_t a = lb * R, e = a + R;
-  if (a > n) a = n;
-  if (e > n) e = n;
-  *r0 = a;
-  *r1 = e;
+// Rows are visited in tiles of kNT consecutive rows; tile t belongs to
+// logical workgroup t mod gridDim.x (grid-stride sweeps).  At any moment
+// the whole grid works inside one window of gridDim.x tiles, which keeps
+// the slot-major ELL streams DRAM-page friendly, and each XCD owns a
+// contiguous part of every window.  Every PCG kernel uses this ownership,
+// so a row's vector entries are written and re-read on the same XCD.
+template <class F>
+__device__ inline void for_my_rows(int64_t n, F&& f) {
+  const int64_t stride = int64_t(gridDim.x) * kNT;
+  for (int64_t base = int64_t(logical_block()) * kNT; base < n; base += stride) {
+    const int64_t i = base + threadIdx.x;
+    if (i < n) f(i);
+  }
 }
 
 // One row of a slot-major ELL matrix, U slots in flight; slots are summed in
@@ -92,46 +94,47 @@ __device__ inline void for_rows_spmv(int64_t n, const T* __restrict__ val,
                                      const int32_t* __restrict__ col,
                                      const int32_t* __restrict__ rownnz,
                                      const T* __restrict__ x, F&& f) {
-  int64_t r0, r1;
-  row_range(n, &r0, &r1);
   if constexpr (V <= kThreadU8) {
     constexpr int U = V == kThreadU1 ? 1 : (V == kThreadU4 ? 4 : 8);
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += kNT)
-      f(i, row_dot<U, T>(i, n, val, col, rownnz[i], x));
+    for_my_rows(n, [&](int64_t i) { f(i, row_dot<U, T>(i, n, val, col, rownnz[i], x)); });
   } else {
+    // the workgroup's kNT-row tile is covered in SUB passes of RPB rows
     constexpr int SUB = V == kTile2 ? 2 : 4;
     constexpr int RPB = kNT / SUB;  // rows per pass
     __shared__ T sh_part[SUB][RPB];
     const int rl = threadIdx.x % RPB, sub = threadIdx.x / RPB;
-    for (int64_t base = r0; base < r1; base += RPB) {
-      const int64_t i = base + rl;
-      T acc = T(0);
-      if (i < r1) {
-        const int32_t rn = rownnz[i];
-        int32_t s = sub;
-        for (; s + SUB < rn; s += 2 * SUB) {
-          const int64_t i0 = i + int64_t(s) * n, i1 = i + int64_t(s + SUB) * n;
-          const T v0 = __builtin_nontemporal_load(val + i0);
-          const T v1 = __builtin_nontemporal_load(val + i1);
-          const int32_t c0 = __builtin_nontemporal_load(col + i0);
-          const int32_t c1 = __builtin_nontemporal_load(col + i1);
-          acc += v0 * x[c0];
-          acc += v1 * x[c1];
+    const int64_t stride = int64_t(gridDim.x) * kNT;
+    for (int64_t tile = int64_t(logical_block()) * kNT; tile < n; tile += stride) {
+      for (int pass = 0; pass < SUB; ++pass) {
+        const int64_t i = tile + pass * RPB + rl;
+        T acc = T(0);
+        if (i < n) {
+          const int32_t rn = rownnz[i];
+          int32_t s = sub;
+          for (; s + SUB < rn; s += 2 * SUB) {
+            const int64_t i0 = i + int64_t(s) * n, i1 = i + int64_t(s + SUB) * n;
+            const T v0 = __builtin_nontemporal_load(val + i0);
+            const T v1 = __builtin_nontemporal_load(val + i1);
+            const int32_t c0 = __builtin_nontemporal_load(col + i0);
+            const int32_t c1 = __builtin_nontemporal_load(col + i1);
+            acc += v0 * x[c0];
+            acc += v1 * x[c1];
+          }
+          for (; s < rn; s += SUB) {
+            const int64_t i0 = i + int64_t(s) * n;
+            acc += __builtin_nontemporal_load(val + i0) * x[__builtin_nontemporal_load(col + i0)];
+          }
         }
-        for (; s < rn; s += SUB) {
-          const int64_t i0 = i + int64_t(s) * n;
-          acc += __builtin_nontemporal_load(val + i0) * x[__builtin_nontemporal_load(col + i0)];
-        }
-      }
-      sh_part[sub][rl] = acc;
-      __syncthreads();
-      if (sub == 0 && i < r1) {
-        T t = sh_part[0][rl];
+        sh_part[sub][rl] = acc;
+        __syncthreads();
+        if (sub == 0 && i < n) {
+          T t = sh_part[0][rl];
 #pragma unroll
-        for (int k = 1; k < SUB; ++k) t += sh_part[k][rl];
-        f(i, t);
+          for (int k = 1; k < SUB; ++k) t += sh_part[k][rl];
+          f(i, t);
+        }
+        __syncthreads();
       }
-      __syncthreads();
     }
   }
 }

@@ -43,10 +43,8 @@ __global__ void k_to_f32(int64_t count, const double* __restrict__ in, float* __
 
 __global__ __launch_bounds__(kNT) void k_m_init(MixedBuffers B, int use_x0) {
   __shared__ double sh[kNT / 64];
-  int64_t r0, r1;
-  row_range(B.n, &r0, &r1);
   double rz = 0, rr = 0, bb = 0;
-  for (int64_t i = r0 + threadIdx.x; i < r1; i += kNT) {
+  for_my_rows(B.n, [&](int64_t i) {
     const double bi = B.b[i];
     double ri;
     if (use_x0) {
@@ -63,7 +61,7 @@ __global__ __launch_bounds__(kNT) void k_m_init(MixedBuffers B, int use_x0) {
     rz += double(r32) * double(z32);
     rr += ri * ri;
     bb += bi * bi;
-  }
+  });
   rz = block_sum(rz, sh);
   rr = block_sum(rr, sh);
   bb = block_sum(bb, sh);
@@ -125,10 +123,8 @@ __global__ __launch_bounds__(kNT) void k_m_update(MixedBuffers B, int parity) {
     return;
   }
   const double alpha = st->rho[parity] / pq;
-  int64_t r0, r1;
-  row_range(B.n, &r0, &r1);
   double rz = 0, rr = 0;
-  for (int64_t i = r0 + threadIdx.x; i < r1; i += kNT) {
+  for_my_rows(B.n, [&](int64_t i) {
     const double pi = double(B.p32[i]);
     B.x[i] += alpha * pi;
     const float r32 = float(double(B.r32[i]) - alpha * double(B.q32[i]));
@@ -137,7 +133,7 @@ __global__ __launch_bounds__(kNT) void k_m_update(MixedBuffers B, int parity) {
     B.z32[i] = z32;
     rz += double(r32) * double(z32);
     rr += double(r32) * double(r32);
-  }
+  });
   rz = block_sum(rz, sh);
   rr = block_sum(rr, sh);
   if (threadIdx.x == 0) {
@@ -196,10 +192,9 @@ __global__ __launch_bounds__(kNT) void k_m_direction(MixedBuffers B, int parity)
   }
   if (!converged) {
     const double beta = rz / st->rho[parity];
-    int64_t r0, r1;
-    row_range(B.n, &r0, &r1);
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += kNT)
+    for_my_rows(B.n, [&](int64_t i) {
       B.p32[i] = float(double(B.z32[i]) + beta * double(B.p32[i]));
+    });
   }
   __syncthreads();
   if (blockIdx.x == 0 && threadIdx.x == 0) {

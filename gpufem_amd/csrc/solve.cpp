@@ -289,16 +289,31 @@ int gf_pcg(gf_ctx* ctx, gf_ell* m, const double* b, double* x, int32_t nrhs,
   GF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
   std::vector<PcgHostState> hs(static_cast<size_t>(nrhs));
   std::vector<int> refinements(static_cast<size_t>(nrhs), 0);
+  int fallbacks = 0;
   for (int k = 0; k < nrhs; ++k) {
     GF_HIP_CHECK(ctx, hipMemcpyAsync(P->wb, Bd + k * n, vb, hipMemcpyDeviceToDevice, ctx->stream));
     if (opts.use_x0)
       GF_HIP_CHECK(ctx, hipMemcpyAsync(P->wx, Xd + k * n, vb, hipMemcpyDeviceToDevice, ctx->stream));
     PcgHostState h{};
-    if (opts.precision == GF_PREC_MIXED)
+    if (opts.precision == GF_PREC_MIXED) {
       st = run_mixed(ctx, m, opts, &h, &refinements[size_t(k)]);
-    else
+      if (st) return st;
+      const bool ok = h.status == GF_OK && h.rr <= opts.rtol * opts.rtol * h.bnorm2;
+      if (!ok) {
+        // FP32 rounding can leave a badly conditioned operator indefinite:
+        // re-solve this right-hand side in FP64 from zero
+        const int mixed_it = h.it;
+        PcgBuffers Bj = make_buffers(m, m->d_dinv);
+        st = run_pcg(ctx, m, Bj, sched == GF_SCHED_PERSISTENT ? sched : GF_SCHED_KERNELS, 1, 0,
+                     opts.rtol, opts.maxit, &h);
+        if (st) return st;
+        h.it += mixed_it;
+        ++fallbacks;
+      }
+    } else {
       st = run_pcg(ctx, m, B, sched, key, opts.use_x0, opts.rtol, opts.maxit, &h);
-    if (st) return st;
+      if (st) return st;
+    }
     GF_HIP_CHECK(ctx, launch_true_residual(ctx->stream, B, P->res + k));
     GF_HIP_CHECK(ctx, hipMemcpyAsync(Xd + k * n, P->wx, vb, hipMemcpyDeviceToDevice, ctx->stream));
     hs[size_t(k)] = h;
@@ -331,6 +346,7 @@ int gf_pcg(gf_ctx* ctx, gf_ell* m, const double* b, double* x, int32_t nrhs,
   }
   S.ms_device = ms;
   S.ms_host = now_ms() - h0;
+  S.fallbacks = fallbacks;
   if (stats) *stats = S;
   if (S.status == GF_ERR_BREAKDOWN)
     return set_error(ctx, GF_ERR_BREAKDOWN, "gf_pcg: breakdown (p.Kp <= 0 or NaN)");

@@ -85,10 +85,8 @@ __device__ inline double phase_spmv_pq(const PcgBuffers& B) {
 
 __device__ inline void phase_update(const PcgBuffers& B, double alpha, double* rz_out,
                                     double* rr_out) {
-  int64_t r0, r1;
-  row_range(B.n, &r0, &r1);
   double rz = 0, rr = 0;
-  for (int64_t i = r0 + threadIdx.x; i < r1; i += kNT) {
+  for_my_rows(B.n, [&](int64_t i) {
     B.x[i] += alpha * B.p[i];
     const double ri = B.r[i] - alpha * B.q[i];
     const double zi = B.dinv ? B.dinv[i] * ri : ri;
@@ -96,24 +94,20 @@ __device__ inline void phase_update(const PcgBuffers& B, double alpha, double* r
     B.z[i] = zi;
     rz += ri * zi;
     rr += ri * ri;
-  }
+  });
   *rz_out = rz;
   *rr_out = rr;
 }
 
 __device__ inline void phase_direction(const PcgBuffers& B, double beta) {
-  int64_t r0, r1;
-  row_range(B.n, &r0, &r1);
-  for (int64_t i = r0 + threadIdx.x; i < r1; i += kNT) B.p[i] = B.z[i] + beta * B.p[i];
+  for_my_rows(B.n, [&](int64_t i) { B.p[i] = B.z[i] + beta * B.p[i]; });
 }
 
 // r = b - q (use_x0) or x = 0, r = b; z = M^-1 r; p = z; partials r.z, r.r, b.b
 __global__ __launch_bounds__(kNT) void k_pcg_init(PcgBuffers B, int use_x0) {
   __shared__ double sh[kNT / 64];
-  int64_t r0, r1;
-  row_range(B.n, &r0, &r1);
   double rz = 0, rr = 0, bb = 0;
-  for (int64_t i = r0 + threadIdx.x; i < r1; i += kNT) {
+  for_my_rows(B.n, [&](int64_t i) {
     const double bi = B.b[i];
     double ri;
     if (use_x0) {
@@ -129,7 +123,7 @@ __global__ __launch_bounds__(kNT) void k_pcg_init(PcgBuffers B, int use_x0) {
     rz += ri * zi;
     rr += ri * ri;
     bb += bi * bi;
-  }
+  });
   rz = block_sum(rz, sh);
   rr = block_sum(rr, sh);
   bb = block_sum(bb, sh);
@@ -213,9 +207,11 @@ __global__ __launch_bounds__(kNT) void k_direction(PcgBuffers B, int parity) {
 constexpr long kSpinLimit = 1L << 24;  // ~1 s of polling: a barrier never takes that long
 
 __device__ inline bool grid_barrier(GridBarrier* bar, int* s_flag) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  // the workgroup barrier makes every wave's stores complete; one agent-scope
+  // release (L2 write-back) then publishes them for the other XCDs
   __syncthreads();
   if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     const unsigned nb = gridDim.x;
     const unsigned g = __hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned arrived =
@@ -235,9 +231,9 @@ __device__ inline bool grid_barrier(GridBarrier* bar, int* s_flag) {
       }
     }
     *s_flag = __hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return *s_flag == 0;
 }
 
@@ -313,15 +309,13 @@ __global__ void k_barrier_reset(GridBarrier* bar) {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kNT) void k_resid_partial(PcgBuffers B) {
   __shared__ double sh[kNT / 64];
-  int64_t r0, r1;
-  row_range(B.n, &r0, &r1);
   double s = 0.0, bb = 0.0;
-  for (int64_t i = r0 + threadIdx.x; i < r1; i += kNT) {
+  for_my_rows(B.n, [&](int64_t i) {
     const double bi = B.b[i];
     const double d = bi - B.q[i];
     s += d * d;
     bb += bi * bi;
-  }
+  });
   s = block_sum(s, sh);
   bb = block_sum(bb, sh);
   if (threadIdx.x == 0) {

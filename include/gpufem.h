@@ -199,6 +199,9 @@ typedef struct gf_solve_stats {
   double true_relres;   /* max over RHS of ||b - K x||/||b|| recomputed in FP64 */
   double ms_device;     /* event time of the solve on the stream */
   double ms_host;       /* host wall time of the call */
+  int32_t fallbacks;    /* right-hand sides re-solved by zero-start FP64 PCG after
+                           the mixed-precision iteration broke down or stalled */
+  int32_t pad_;
 } gf_solve_stats;
 
 void gf_solve_opts_default(gf_solve_opts* o);
