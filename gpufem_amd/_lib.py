@@ -49,7 +49,8 @@ class EllInfo(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("precond", C.c_int32), ("precision", C.c_int32), ("schedule", C.c_int32),
                 ("use_x0", C.c_int32), ("maxit", C.c_int32), ("device_ptrs", C.c_int32),
-                ("rtol", C.c_double), ("inner_rtol", C.c_double)]
+                ("rtol", C.c_double), ("inner_rtol", C.c_double), ("batched", C.c_int32),
+                ("pad_", C.c_int32)]
 
 
 class SolveStats(C.Structure):

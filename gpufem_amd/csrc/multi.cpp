@@ -1,0 +1,97 @@
+// Host driver of the batched PCG (kernels in multi.hip).
+#include <string>
+
+#include "multi.hpp"
+#include "solve_plan.hpp"
+
+struct gf_multi_ws {
+  int64_t n = 0;
+  int S = 0;
+  double *b = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  double *part1 = nullptr, *part2 = nullptr, *res = nullptr;
+  void* st = nullptr;
+};
+
+namespace gf {
+
+int set_error(gf_ctx* ctx, int code, const std::string& msg);
+
+namespace {
+constexpr int kChunk = 16;
+}
+
+void release_multi(gf_solve_plan* P) {
+  gf_multi_ws* W = P->multi;
+  if (!W) return;
+  double* v[] = {W->b, W->x, W->r, W->z, W->p, W->q, W->part1, W->part2, W->res};
+  for (double* d : v)
+    if (d) (void)hipFree(d);
+  if (W->st) (void)hipFree(W->st);
+  delete W;
+  P->multi = nullptr;
+}
+
+int run_multi(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, const double* Bd, double* Xd,
+              int ncols, MultiHostState* h, double* true_res) {
+  gf_solve_plan* P = m->plan;
+  const int S = multi_width(ncols);
+  if (S == 0) return set_error(ctx, GF_ERR_UNSUPPORTED, "gf_pcg: at most 64 right-hand sides");
+  const int64_t n = m->n;
+  if (!P->multi || P->multi->S != S || P->multi->n != n) {
+    release_multi(P);
+    gf_multi_ws* W = new gf_multi_ws();
+    P->multi = W;
+    W->n = n;
+    W->S = S;
+    const size_t vb = size_t(n) * size_t(S) * sizeof(double);
+    const int nb = multi_blocks(n, S);
+    hipError_t e = hipSuccess;
+    double** v[] = {&W->b, &W->x, &W->r, &W->z, &W->p, &W->q};
+    for (double** d : v)
+      if (e == hipSuccess) e = hipMalloc(d, vb);
+    if (e == hipSuccess) e = hipMalloc(&W->part1, size_t(S) * nb * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&W->part2, size_t(3 * S) * nb * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&W->res, 64 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&W->st, multi_state_bytes(S));
+    if (e != hipSuccess) {
+      release_multi(P);
+      return set_error(ctx, GF_ERR_NOMEM, std::string("gf_pcg (batched): workspace: ") + hipGetErrorString(e));
+    }
+  }
+  gf_multi_ws* W = P->multi;
+  MultiBuffers B;
+  B.n = n;
+  B.S = S;
+  B.ncols = ncols;
+  B.use_x0 = opts.use_x0;
+  B.nblocks = multi_blocks(n, S);
+  B.val = m->d_val;
+  B.col = m->d_col;
+  B.rownnz = m->d_rownnz;
+  B.dinv = opts.precond == GF_PRECOND_JACOBI ? m->d_dinv : nullptr;
+  B.b = W->b;
+  B.x = W->x;
+  B.r = W->r;
+  B.z = W->z;
+  B.p = W->p;
+  B.q = W->q;
+  B.part1 = W->part1;
+  B.part2 = W->part2;
+  B.st = W->st;
+  GF_HIP_CHECK(ctx, launch_multi_pack(ctx->stream, S, n, ncols, Bd, W->b));
+  if (opts.use_x0) GF_HIP_CHECK(ctx, launch_multi_pack(ctx->stream, S, n, ncols, Xd, W->x));
+  GF_HIP_CHECK(ctx, launch_multi_setup(ctx->stream, B, opts.rtol, opts.maxit));
+  GF_HIP_CHECK(ctx, read_multi_state(ctx->stream, B, h));
+  while (!h->done) {
+    for (int it = 0; it < kChunk; ++it) GF_HIP_CHECK(ctx, launch_multi_iteration(ctx->stream, B));
+    GF_HIP_CHECK(ctx, read_multi_state(ctx->stream, B, h));
+  }
+  GF_HIP_CHECK(ctx, launch_multi_residual(ctx->stream, B, W->res));
+  GF_HIP_CHECK(ctx, launch_multi_unpack(ctx->stream, S, n, ncols, W->x, Xd));
+  GF_HIP_CHECK(ctx, hipMemcpyAsync(true_res, W->res, size_t(ncols) * sizeof(double),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+  GF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return GF_OK;
+}
+
+}  // namespace gf

@@ -1,0 +1,502 @@
+// Batched Jacobi-PCG over S load cases with one ELL SpMM per iteration.
+//
+// The S right-hand sides advance together, each with its own alpha, beta
+// and stopping test (the columns of a block Krylov solve without the
+// cross-column Gram terms), so every iteration reads the matrix once for
+// all S columns instead of once per column.  Vectors are stored row-major,
+// V[i*S + k], so the S entries a gathered column contributes are contiguous.
+// A workgroup is 256/S row groups of S lanes; lane k of a group handles
+// load case k of its rows.  A converged column keeps alpha = beta = 0 from
+// then on, freezing its solution.
+#include "device_common.hpp"
+#include "gf_internal.hpp"
+#include "launch.hpp"
+#include "multi.hpp"
+
+namespace gf {
+
+template <int S>
+struct MultiState {
+  double rho[S];
+  double alpha[S];
+  double beta[S];
+  double bnorm2[S];
+  double rr[S];
+  int32_t col_it[S];    // iterations each column took
+  int32_t col_done[S];  // 1: converged (or empty) column
+  int32_t col_status[S];
+  int32_t it, done, status, maxit;
+  double rtol2;
+};
+
+namespace {
+
+using namespace dev;
+
+template <int S>
+__device__ inline void groups(int* g, int* k) {
+  *k = threadIdx.x % S;
+  *g = threadIdx.x / S;
+}
+
+// rows of the workgroup: tiles of 256/S*8 rows would change the ownership of
+// the single-RHS kernels; here a tile is the group count times 8 rows.
+template <int S>
+struct Tiles {
+  static constexpr int G = kNT / S;   // row groups per workgroup
+  static constexpr int RT = 8 * G;    // rows per tile
+};
+
+template <int S, class F>
+__device__ inline void for_group_rows(int64_t n, F&& f) {
+  int g, k;
+  groups<S>(&g, &k);
+  const int64_t stride = int64_t(gridDim.x) * Tiles<S>::RT;
+  for (int64_t base = int64_t(logical_block()) * Tiles<S>::RT; base < n; base += stride) {
+    const int64_t end = base + Tiles<S>::RT < n ? base + Tiles<S>::RT : n;
+    for (int64_t i = base + g; i < end; i += Tiles<S>::G) f(i, k);
+  }
+}
+
+// per-column sums over the workgroup: every thread of column k gets the sum
+template <int S>
+__device__ inline double column_sum(double v, double* sh) {
+  int g, k;
+  groups<S>(&g, &k);
+  __syncthreads();
+  sh[g * S + k] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int j = 0; j < Tiles<S>::G; ++j) t += sh[j * S + k];
+  return t;
+}
+
+// Single workgroup: sums[k] = sum over b of part[(base + k) * nb + b] for the S
+// columns, in a fixed order (row group g takes b = g, g + G, ...; groups are
+// then added in order).  Result in LDS out[S], visible after the call.
+template <int S>
+__device__ inline void reduce_columns(const double* part, int nb, int base, double* sh,
+                                      double* out) {
+  int g, k;
+  groups<S>(&g, &k);
+  double t = 0.0;
+  for (int bb = g; bb < nb; bb += Tiles<S>::G) t += part[int64_t(base + k) * nb + bb];
+  __syncthreads();
+  sh[g * S + k] = t;
+  __syncthreads();
+  if (g == 0) {
+    double u = 0.0;
+    for (int j = 0; j < Tiles<S>::G; ++j) u += sh[j * S + k];
+    out[k] = u;
+  }
+  __syncthreads();
+}
+
+template <int S>
+__global__ __launch_bounds__(kNT) void k_b_init(MultiBuffers B) {
+  __shared__ double sh[kNT];
+  double rz = 0, rr = 0, bb = 0;
+  for_group_rows<S>(B.n, [&](int64_t i, int k) {
+    const int64_t o = i * S + k;
+    const double bi = B.b[o];
+    const double ri = B.use_x0 ? bi - B.q[o] : bi;
+    if (!B.use_x0) B.x[o] = 0.0;
+    const double zi = B.dinv ? B.dinv[i] * ri : ri;
+    B.r[o] = ri;
+    B.z[o] = zi;
+    B.p[o] = zi;
+    rz += ri * zi;
+    rr += ri * ri;
+    bb += bi * bi;
+  });
+  rz = column_sum<S>(rz, sh);
+  rr = column_sum<S>(rr, sh);
+  bb = column_sum<S>(bb, sh);
+  if (threadIdx.x < S) {
+    const int k = threadIdx.x, nb = gridDim.x;
+    B.part2[int64_t(k) * nb + blockIdx.x] = rz;
+    B.part2[int64_t(S + k) * nb + blockIdx.x] = rr;
+    B.part2[int64_t(2 * S + k) * nb + blockIdx.x] = bb;
+  }
+}
+
+template <int S>
+__global__ __launch_bounds__(kNT) void k_b_init_finish(MultiBuffers B, int nb, int ncols,
+                                                       double rtol2, int maxit) {
+  MultiState<S>* st = static_cast<MultiState<S>*>(B.st);
+  __shared__ double sh[kNT];
+  __shared__ double rz[S], rr[S], bb[S];
+  reduce_columns<S>(B.part2, nb, 0, sh, rz);
+  reduce_columns<S>(B.part2, nb, S, sh, rr);
+  reduce_columns<S>(B.part2, nb, 2 * S, sh, bb);
+  if (threadIdx.x < S) {
+    const int k = threadIdx.x;
+    st->rho[k] = rz[k];
+    st->alpha[k] = 0.0;
+    st->beta[k] = 0.0;
+    st->bnorm2[k] = bb[k];
+    st->rr[k] = rr[k];
+    st->col_it[k] = 0;
+    st->col_status[k] = GF_OK;
+    st->col_done[k] = (k >= ncols) || (rr[k] <= rtol2 * bb[k]) || maxit <= 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int all = 1;
+    for (int j = 0; j < S; ++j) all &= st->col_done[j];
+    st->it = 0;
+    st->done = all;
+    st->status = GF_OK;
+    st->maxit = maxit;
+    st->rtol2 = rtol2;
+  }
+}
+
+template <int S>
+__global__ __launch_bounds__(kNT) void k_b_spmm_pq(MultiBuffers B) {
+  const MultiState<S>* st = static_cast<const MultiState<S>*>(B.st);
+  if (st->done) return;
+  __shared__ double sh[kNT];
+  double pq = 0.0;
+  const int64_t n = B.n;
+  for_group_rows<S>(n, [&](int64_t i, int k) {
+    const int32_t rn = B.rownnz[i];
+    double acc = 0.0;
+    int32_t s = 0;
+    for (; s + 4 <= rn; s += 4) {
+      double v[4];
+      int32_t c[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int64_t idx = i + int64_t(s + u) * n;
+        v[u] = B.val[idx];
+        c[u] = B.col[idx];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc += v[u] * B.p[int64_t(c[u]) * S + k];
+    }
+    for (; s < rn; ++s) {
+      const int64_t idx = i + int64_t(s) * n;
+      acc += B.val[idx] * B.p[int64_t(B.col[idx]) * S + k];
+    }
+    B.q[i * S + k] = acc;
+    pq += B.p[i * S + k] * acc;
+  });
+  pq = column_sum<S>(pq, sh);
+  if (threadIdx.x < S) B.part1[int64_t(threadIdx.x) * gridDim.x + blockIdx.x] = pq;
+}
+
+// alpha_k = rho_k / p_k.q_k (0 for finished columns; a column with
+// p.q <= 0 breaks down and stops)
+template <int S>
+__global__ __launch_bounds__(kNT) void k_b_alpha(MultiBuffers B, int nb) {
+  MultiState<S>* st = static_cast<MultiState<S>*>(B.st);
+  if (st->done) return;
+  __shared__ double sh[kNT];
+  __shared__ double pq[S];
+  reduce_columns<S>(B.part1, nb, 0, sh, pq);
+  if (threadIdx.x < S) {
+    const int k = threadIdx.x;
+    double a = 0.0;
+    if (!st->col_done[k]) {
+      if (pq[k] > 0.0) {
+        a = st->rho[k] / pq[k];
+      } else {
+        st->col_status[k] = GF_ERR_BREAKDOWN;
+        st->col_done[k] = 1;
+      }
+    }
+    st->alpha[k] = a;
+  }
+}
+
+template <int S>
+__global__ __launch_bounds__(kNT) void k_b_update(MultiBuffers B) {
+  MultiState<S>* st = static_cast<MultiState<S>*>(B.st);
+  if (st->done) return;
+  __shared__ double sh[kNT];
+  __shared__ double s_alpha[S];
+  if (threadIdx.x < S) s_alpha[threadIdx.x] = st->alpha[threadIdx.x];
+  __syncthreads();
+  const int nb = gridDim.x;
+  double rz = 0, rr = 0;
+  for_group_rows<S>(B.n, [&](int64_t i, int k) {
+    const int64_t o = i * S + k;
+    const double a = s_alpha[k];
+    B.x[o] += a * B.p[o];
+    const double ri = B.r[o] - a * B.q[o];
+    const double zi = B.dinv ? B.dinv[i] * ri : ri;
+    B.r[o] = ri;
+    B.z[o] = zi;
+    rz += ri * zi;
+    rr += ri * ri;
+  });
+  rz = column_sum<S>(rz, sh);
+  rr = column_sum<S>(rr, sh);
+  if (threadIdx.x < S) {
+    B.part2[int64_t(threadIdx.x) * nb + blockIdx.x] = rz;
+    B.part2[int64_t(S + threadIdx.x) * nb + blockIdx.x] = rr;
+  }
+}
+
+// beta_k, the per-column stopping test and the iteration counters
+template <int S>
+__global__ __launch_bounds__(kNT) void k_b_beta(MultiBuffers B, int nb) {
+  MultiState<S>* st = static_cast<MultiState<S>*>(B.st);
+  if (st->done) return;
+  __shared__ double sh[kNT];
+  __shared__ double rz[S], rr[S];
+  reduce_columns<S>(B.part2, nb, 0, sh, rz);
+  reduce_columns<S>(B.part2, nb, S, sh, rr);
+  const int it = st->it + 1;
+  if (threadIdx.x < S) {
+    const int k = threadIdx.x;
+    double beta = 0.0;
+    if (!st->col_done[k]) {
+      beta = rz[k] / st->rho[k];
+      st->rho[k] = rz[k];
+      st->rr[k] = rr[k];
+      st->col_it[k] = it;
+      if (rr[k] <= st->rtol2 * st->bnorm2[k] || it >= st->maxit) {
+        st->col_done[k] = 1;
+        beta = 0.0;
+      }
+    }
+    st->beta[k] = beta;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int all = 1;
+    for (int j = 0; j < S; ++j) all &= st->col_done[j];
+    st->it = it;
+    if (all) st->done = 1;
+  }
+}
+
+template <int S>
+__global__ __launch_bounds__(kNT) void k_b_direction(MultiBuffers B) {
+  MultiState<S>* st = static_cast<MultiState<S>*>(B.st);
+  if (st->done) return;
+  __shared__ double s_beta[S];
+  if (threadIdx.x < S) s_beta[threadIdx.x] = st->beta[threadIdx.x];
+  __syncthreads();
+  for_group_rows<S>(B.n, [&](int64_t i, int k) {
+    const int64_t o = i * S + k;
+    B.p[o] = B.z[o] + s_beta[k] * B.p[o];
+  });
+}
+
+// transposes between column-after-column and row-major (i*S + k) layouts
+template <int S>
+__global__ void k_b_pack(int64_t n, int ncols, const double* __restrict__ cols,
+                         double* __restrict__ rows) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n * S;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t i = t / S;
+    const int k = int(t % S);
+    rows[t] = k < ncols ? cols[int64_t(k) * n + i] : 0.0;
+  }
+}
+
+template <int S>
+__global__ void k_b_unpack(int64_t n, int ncols, const double* __restrict__ rows,
+                           double* __restrict__ cols) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n * ncols;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int k = int(t / n);
+    const int64_t i = t % n;
+    cols[t] = rows[i * S + k];
+  }
+}
+
+// per-column ||b - K x||^2 and ||b||^2 (q holds K x)
+template <int S>
+__global__ __launch_bounds__(kNT) void k_b_resid(MultiBuffers B) {
+  __shared__ double sh[kNT];
+  double s = 0, bb = 0;
+  for_group_rows<S>(B.n, [&](int64_t i, int k) {
+    const int64_t o = i * S + k;
+    const double d = B.b[o] - B.q[o];
+    s += d * d;
+    bb += B.b[o] * B.b[o];
+  });
+  s = column_sum<S>(s, sh);
+  bb = column_sum<S>(bb, sh);
+  if (threadIdx.x < S) {
+    B.part1[int64_t(threadIdx.x) * gridDim.x + blockIdx.x] = s;
+    B.part2[int64_t(threadIdx.x) * gridDim.x + blockIdx.x] = bb;
+  }
+}
+
+template <int S>
+__global__ __launch_bounds__(kNT) void k_b_resid_finish(MultiBuffers B, int nb, double* out) {
+  __shared__ double sh[kNT];
+  __shared__ double ss[S], bb[S];
+  reduce_columns<S>(B.part1, nb, 0, sh, ss);
+  reduce_columns<S>(B.part2, nb, 0, sh, bb);
+  if (threadIdx.x < S) {
+    const int k = threadIdx.x;
+    out[k] = bb[k] > 0.0 ? sqrt(ss[k] / bb[k]) : sqrt(ss[k]);
+  }
+}
+
+// K x for the true residual (and the warm start): the same SpMM on x
+template <int S>
+__global__ __launch_bounds__(kNT) void k_b_spmm(MultiBuffers B, const double* __restrict__ xin,
+                                                double* __restrict__ yout) {
+  const int64_t n = B.n;
+  for_group_rows<S>(n, [&](int64_t i, int k) {
+    const int32_t rn = B.rownnz[i];
+    double acc = 0.0;
+    for (int32_t s = 0; s < rn; ++s) {
+      const int64_t idx = i + int64_t(s) * n;
+      acc += B.val[idx] * xin[int64_t(B.col[idx]) * S + k];
+    }
+    yout[i * S + k] = acc;
+  });
+}
+
+inline int grid_for(int64_t work, int threads, int cap) {
+  int64_t b = (work + threads - 1) / threads;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return int(b);
+}
+
+template <int S>
+hipError_t setup_s(hipStream_t s, const MultiBuffers& B, double rtol, int maxit) {
+  hipError_t e;
+  if (B.use_x0) {
+    hipLaunchKernelGGL(k_b_spmm<S>, dim3(B.nblocks), dim3(kNT), 0, s, B, B.x, B.q);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_b_init<S>, dim3(B.nblocks), dim3(kNT), 0, s, B);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_b_init_finish<S>, dim3(1), dim3(kNT), 0, s, B, B.nblocks, B.ncols,
+                     rtol * rtol, maxit);
+  return hipGetLastError();
+}
+
+template <int S>
+hipError_t iteration_s(hipStream_t s, const MultiBuffers& B) {
+  hipError_t e;
+  const dim3 g(B.nblocks), t(kNT);
+  hipLaunchKernelGGL(k_b_spmm_pq<S>, g, t, 0, s, B);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_b_alpha<S>, dim3(1), t, 0, s, B, B.nblocks);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_b_update<S>, g, t, 0, s, B);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_b_beta<S>, dim3(1), t, 0, s, B, B.nblocks);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_b_direction<S>, g, t, 0, s, B);
+  return hipGetLastError();
+}
+
+template <int S>
+hipError_t residual_s(hipStream_t s, const MultiBuffers& B, double* out) {
+  hipError_t e;
+  hipLaunchKernelGGL(k_b_spmm<S>, dim3(B.nblocks), dim3(kNT), 0, s, B, B.x, B.q);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_b_resid<S>, dim3(B.nblocks), dim3(kNT), 0, s, B);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_b_resid_finish<S>, dim3(1), dim3(kNT), 0, s, B, B.nblocks, out);
+  return hipGetLastError();
+}
+
+template <int S>
+hipError_t read_s(hipStream_t s, const MultiBuffers& B, MultiHostState* h) {
+  MultiState<S> st;
+  hipError_t e = hipMemcpyAsync(&st, B.st, sizeof(st), hipMemcpyDeviceToHost, s);
+  if (e != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  h->it = st.it;
+  h->done = st.done;
+  for (int k = 0; k < B.ncols; ++k) {
+    h->col_it[k] = st.col_it[k];
+    h->col_status[k] = st.col_status[k];
+    h->rr[k] = st.rr[k];
+    h->bnorm2[k] = st.bnorm2[k];
+  }
+  return hipSuccess;
+}
+
+}  // namespace
+
+int multi_width(int ncols) {
+  if (ncols <= 8) return 8;
+  if (ncols <= 16) return 16;
+  if (ncols <= 32) return 32;
+  if (ncols <= 64) return 64;
+  return 0;
+}
+
+size_t multi_state_bytes(int S) {
+  switch (S) {
+    case 8: return sizeof(MultiState<8>);
+    case 16: return sizeof(MultiState<16>);
+    case 32: return sizeof(MultiState<32>);
+    default: return sizeof(MultiState<64>);
+  }
+}
+
+int multi_blocks(int64_t n, int S) {
+  const int64_t rt = 8 * (kNT / S);
+  int64_t nb = (n + rt - 1) / rt;
+  if (nb > kMaxPcgBlocks) nb = kMaxPcgBlocks;
+  if (nb >= 8) nb &= ~int64_t(7);
+  return int(nb < 1 ? 1 : nb);
+}
+
+#define GF_MULTI_SWITCH(S, FN, ...)            \
+  switch (S) {                                 \
+    case 8: return FN<8>(__VA_ARGS__);         \
+    case 16: return FN<16>(__VA_ARGS__);       \
+    case 32: return FN<32>(__VA_ARGS__);       \
+    case 64: return FN<64>(__VA_ARGS__);       \
+    default: return hipErrorInvalidValue;      \
+  }
+
+hipError_t launch_multi_pack(hipStream_t s, int S, int64_t n, int ncols, const double* cols,
+                             double* rows) {
+  const dim3 g(grid_for(n * S, 256, 16384)), b(256);
+  switch (S) {
+    case 8: hipLaunchKernelGGL(k_b_pack<8>, g, b, 0, s, n, ncols, cols, rows); break;
+    case 16: hipLaunchKernelGGL(k_b_pack<16>, g, b, 0, s, n, ncols, cols, rows); break;
+    case 32: hipLaunchKernelGGL(k_b_pack<32>, g, b, 0, s, n, ncols, cols, rows); break;
+    case 64: hipLaunchKernelGGL(k_b_pack<64>, g, b, 0, s, n, ncols, cols, rows); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_multi_unpack(hipStream_t s, int S, int64_t n, int ncols, const double* rows,
+                               double* cols) {
+  const dim3 g(grid_for(n * ncols, 256, 16384)), b(256);
+  switch (S) {
+    case 8: hipLaunchKernelGGL(k_b_unpack<8>, g, b, 0, s, n, ncols, rows, cols); break;
+    case 16: hipLaunchKernelGGL(k_b_unpack<16>, g, b, 0, s, n, ncols, rows, cols); break;
+    case 32: hipLaunchKernelGGL(k_b_unpack<32>, g, b, 0, s, n, ncols, rows, cols); break;
+    case 64: hipLaunchKernelGGL(k_b_unpack<64>, g, b, 0, s, n, ncols, rows, cols); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_multi_setup(hipStream_t s, const MultiBuffers& B, double rtol, int maxit) {
+  GF_MULTI_SWITCH(B.S, setup_s, s, B, rtol, maxit)
+}
+
+hipError_t launch_multi_iteration(hipStream_t s, const MultiBuffers& B) {
+  GF_MULTI_SWITCH(B.S, iteration_s, s, B)
+}
+
+hipError_t launch_multi_residual(hipStream_t s, const MultiBuffers& B, double* out) {
+  GF_MULTI_SWITCH(B.S, residual_s, s, B, out)
+}
+
+hipError_t read_multi_state(hipStream_t s, const MultiBuffers& B, MultiHostState* h) {
+  GF_MULTI_SWITCH(B.S, read_s, s, B, h)
+}
+
+}  // namespace gf

@@ -1,0 +1,54 @@
+// Batched (multi right-hand-side) Jacobi-PCG with one ELL SpMM per iteration
+// (kernels in multi.hip, host driver in multi.cpp).
+#pragma once
+
+#include "gf_internal.hpp"
+#include "launch.hpp"
+
+struct gf_solve_plan;
+
+namespace gf {
+
+struct MultiBuffers {
+  int64_t n;
+  int S;      // columns in the row-major layout (8, 16, 32 or 64)
+  int ncols;  // columns actually solved (<= S); the rest stay zero
+  int use_x0;
+  int nblocks;
+  const double* val;
+  const int32_t* col;
+  const int32_t* rownnz;
+  const double* dinv;  // null: no preconditioner
+  const double* b;     // [n][S]
+  double* x;           // [n][S]
+  double *r, *z, *p, *q;
+  double* part1;  // [S][nblocks]
+  double* part2;  // [3S][nblocks]
+  void* st;       // MultiState<S>
+};
+
+struct MultiHostState {
+  int32_t it, done;
+  int32_t col_it[64], col_status[64];
+  double rr[64], bnorm2[64];
+};
+
+int multi_width(int ncols);       // smallest supported S >= ncols (0: too many)
+size_t multi_state_bytes(int S);
+int multi_blocks(int64_t n, int S);
+hipError_t launch_multi_pack(hipStream_t s, int S, int64_t n, int ncols, const double* cols,
+                             double* rows);
+hipError_t launch_multi_unpack(hipStream_t s, int S, int64_t n, int ncols, const double* rows,
+                               double* cols);
+hipError_t launch_multi_setup(hipStream_t s, const MultiBuffers& B, double rtol, int maxit);
+hipError_t launch_multi_iteration(hipStream_t s, const MultiBuffers& B);
+hipError_t launch_multi_residual(hipStream_t s, const MultiBuffers& B, double* out);
+hipError_t read_multi_state(hipStream_t s, const MultiBuffers& B, MultiHostState* h);
+
+// Solves ncols right-hand sides (column after column at Bd / Xd, device
+// pointers) together.  Fills per-column iterations/residuals.
+int run_multi(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, const double* Bd, double* Xd,
+              int ncols, MultiHostState* h, double* true_res);
+void release_multi(gf_solve_plan* P);
+
+}  // namespace gf

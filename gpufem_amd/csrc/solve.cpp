@@ -16,6 +16,7 @@
 #include "gf_internal.hpp"
 #include "launch.hpp"
 #include "mixed.hpp"
+#include "multi.hpp"
 #include "solve_plan.hpp"
 
 namespace gf {
@@ -201,6 +202,7 @@ void release_solve_plan(gf_ell* m) {
   if (P->st) (void)hipFree(P->st);
   if (P->barrier) (void)hipFree(P->barrier);
   release_mixed(P);
+  release_multi(P);
   delete P;
   m->plan = nullptr;
 }
@@ -221,6 +223,7 @@ void gf_solve_opts_default(gf_solve_opts* o) {
   o->device_ptrs = 0;
   o->rtol = 1e-8;
   o->inner_rtol = 1e-4;
+  o->batched = 1;
 }
 
 int gf_ell_tune(gf_ctx* ctx, gf_ell* m, int32_t force_variant, double* ms_per_variant,
@@ -287,6 +290,41 @@ int gf_pcg(gf_ctx* ctx, gf_ell* m, const double* b, double* x, int32_t nrhs,
     Xd = tmp.p + n * nrhs;
   }
   GF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
+  if (nrhs > 1 && opts.batched && opts.precision == GF_PREC_FP64) {
+    MultiHostState mh;
+    std::vector<double> res(static_cast<size_t>(nrhs), 0.0);
+    st = run_multi(ctx, m, opts, Bd, Xd, nrhs, &mh, res.data());
+    if (st) return st;
+    GF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
+    GF_HIP_CHECK(ctx, hipEventSynchronize(ctx->ev_stop));
+    float msb = 0.f;
+    GF_HIP_CHECK(ctx, hipEventElapsedTime(&msb, ctx->ev_start, ctx->ev_stop));
+    if (!opts.device_ptrs) GF_HIP_CHECK(ctx, hipMemcpy(x, Xd, vb * nrhs, hipMemcpyDeviceToHost));
+    gf_solve_stats S{};
+    S.converged = 1;
+    S.status = GF_OK;
+    for (int k = 0; k < nrhs; ++k) {
+      S.iterations = std::max(S.iterations, mh.col_it[k]);
+      const double bb = mh.bnorm2[k];
+      S.relres = std::max(S.relres, bb > 0 ? std::sqrt(mh.rr[k] / bb) : std::sqrt(mh.rr[k]));
+      S.true_relres = std::max(S.true_relres, res[size_t(k)]);
+      if (mh.col_status[k] != GF_OK) {
+        S.status = mh.col_status[k];
+        S.converged = 0;
+      } else if (!(mh.rr[k] <= opts.rtol * opts.rtol * bb)) {
+        if (S.status == GF_OK) S.status = GF_ERR_NOCONV;
+        S.converged = 0;
+      }
+    }
+    S.ms_device = msb;
+    S.ms_host = now_ms() - h0;
+    if (stats) *stats = S;
+    if (S.status == GF_ERR_BREAKDOWN)
+      return set_error(ctx, GF_ERR_BREAKDOWN, "gf_pcg: breakdown (p.Kp <= 0 or NaN)");
+    if (S.status == GF_ERR_NOCONV)
+      return set_error(ctx, GF_ERR_NOCONV, "gf_pcg: iteration limit reached before rtol");
+    return GF_OK;
+  }
   std::vector<PcgHostState> hs(static_cast<size_t>(nrhs));
   std::vector<int> refinements(static_cast<size_t>(nrhs), 0);
   int fallbacks = 0;

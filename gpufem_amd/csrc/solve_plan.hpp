@@ -7,6 +7,7 @@
 #include "launch.hpp"
 
 struct gf_mixed_ws;  // mixed-precision workspace (mixed.cpp)
+struct gf_multi_ws;  // batched multi right-hand-side workspace (multi.cpp)
 
 struct gf_solve_plan {
   int64_t n = 0;
@@ -22,6 +23,7 @@ struct gf_solve_plan {
   void* barrier = nullptr;  // grid barrier of the persistent kernel
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // keyed by preconditioner use
   gf_mixed_ws* mixed = nullptr;
+  gf_multi_ws* multi = nullptr;
 };
 
 namespace gf {

@@ -187,7 +187,10 @@ typedef struct gf_solve_opts {
   int32_t maxit;     /* iteration cap (per right-hand side) */
   int32_t device_ptrs; /* 1: b and x are device pointers */
   double rtol;       /* stop when ||b - K x||_2 <= rtol ||b||_2 */
-  double inner_rtol; /* GF_PREC_MIXED: inner reduction per refinement sweep */
+  double inner_rtol; /* reserved */
+  int32_t batched;   /* nrhs > 1 (FP64): 1 = all right-hand sides advance together
+                        with one ELL SpMM per iteration; 0 = one after another */
+  int32_t pad_;
 } gf_solve_opts;
 
 typedef struct gf_solve_stats {

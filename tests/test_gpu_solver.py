@@ -158,11 +158,34 @@ def test_pcg_multi_rhs_and_warm_start(gpu_ctx):
     gpu_ctx.assemble(mesh, K)
     gpu_ctx.apply_penalty(K, p.fixed_dofs)
     F = P.random_top_loads(p, 3, 99)
-    X, st = gpu_ctx.pcg(K, F)
+    X, st = gpu_ctx.pcg(K, F, batched=0)          # one right-hand side after another
     for k in range(3):
         xk, sk = gpu_ctx.pcg(K, F[k])
         assert np.array_equal(X[k], xk)
     xw, sw = gpu_ctx.pcg(K, F[0], x0=X[0])
+    assert sw.iterations == 0
+
+
+@pytest.mark.parametrize("nrhs", [2, 5, 32])
+def test_batched_pcg_matches_column_solves(gpu_ctx, nrhs):
+    """All load cases advance together (one SpMM per iteration); each column
+    keeps its own scalars, so it follows its single-RHS iteration."""
+    p = P.structured(gf.GF_BRICK20, 3, 3, 2)
+    mesh = gpu_ctx.upload_mesh(p.elem_type, p.ngauss, p.conn, p.coords, p.E, p.nu)
+    K = gpu_ctx.symbolic(mesh)
+    gpu_ctx.assemble(mesh, K)
+    gpu_ctx.apply_penalty(K, p.fixed_dofs)
+    F = P.random_top_loads(p, nrhs, 7)
+    F[nrhs // 2] = 0.0                            # an empty load case converges at once
+    X, st = gpu_ctx.pcg(K, F)
+    assert st.converged and st.true_relres <= 1.05e-8
+    Ko, _ = O.assemble(p.dim, p.ngauss, p.conn, p.coords, p.E, p.nu, skip_zero=False)
+    O.apply_penalty(Ko, p.dim, p.supports)
+    for k in range(nrhs):
+        xo, ito, _, _ = O.pcg_jacobi(Ko, F[k], rtol=1e-8)
+        assert np.linalg.norm(X[k] - xo) <= 1e-9 * max(np.linalg.norm(xo), 1e-300)
+    assert np.all(X[nrhs // 2] == 0)
+    Xw, sw = gpu_ctx.pcg(K, F, x0=X)
     assert sw.iterations == 0
 
 
