@@ -173,18 +173,26 @@ def main():
         "iterations": float(np.mean(iters)), "seconds": float(np.mean(pcg_ms)) / 1e3,
         "true_relres": float(np.max(relres))}}
     if args.compare == "all":
+        # each comparison is capped at a few times the FP64 iteration count
+        cap = int(4 * max(iters)) + 100
         for name, o in [
-                ("fp64_graph", gf.solve_opts(device_ptrs=1, rtol=1e-8,
+                ("fp64_graph", gf.solve_opts(device_ptrs=1, rtol=1e-8, maxit=cap,
                                              schedule=gf.GF_SCHED_GRAPH)),
-                ("fp64_kernels", gf.solve_opts(device_ptrs=1, rtol=1e-8,
+                ("fp64_kernels", gf.solve_opts(device_ptrs=1, rtol=1e-8, maxit=cap,
                                                schedule=gf.GF_SCHED_KERNELS)),
-                ("fp64_persistent", gf.solve_opts(device_ptrs=1, rtol=1e-8,
+                ("fp64_persistent", gf.solve_opts(device_ptrs=1, rtol=1e-8, maxit=cap,
                                                   schedule=gf.GF_SCHED_PERSISTENT)),
-                ("mixed", gf.solve_opts(device_ptrs=1, rtol=1e-8,
+                ("mixed", gf.solve_opts(device_ptrs=1, rtol=1e-8, maxit=cap,
                                         precision=gf.GF_PREC_MIXED))]:
             if name in solves:
                 continue
-            st = solve(o)
+            log("%s: solving ..." % name)
+            try:
+                st = solve(o)
+            except gf.GpufemError as e:
+                log("%s: %s" % (name, e))
+                solves[name] = {"error": str(e)}
+                continue
             solves[name] = {"iterations": st.iterations, "seconds": st.ms_device / 1e3,
                             "true_relres": st.true_relres, "refinements": st.refinements}
             log("%s: %d it, %.3f s, true relres %.2e" % (name, st.iterations,

@@ -77,6 +77,16 @@ def rng_uniform_int(seed, n, lo, hi):
     return out
 
 
+def grf(seed, shape, lengthscale):
+    """Unit-variance Gaussian random field on a cell grid `shape` = (nx, ny[, nz]);
+    returns an array indexed [k, j, i] (x fastest in memory)."""
+    nx, ny = shape[0], shape[1]
+    nz = shape[2] if len(shape) > 2 else 1
+    out = np.empty(nx * ny * nz, dtype=np.float64)
+    check(load().gf_grf(C.c_uint64(seed), nx, ny, nz, float(lengthscale), _ptr(out)))
+    return out.reshape((nz, ny, nx) if len(shape) > 2 else (ny, nx))
+
+
 # ---------------------------------------------------------------------------
 # device objects
 # ---------------------------------------------------------------------------
@@ -257,8 +267,196 @@ class Context:
         st.code = code
         return x, st
 
+    def group(self, problems):
+        """Grouped batch of independent meshes (gpufem_amd.problems.Problem)."""
+        return Group(self, problems)
+
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+class Group:
+    """gf_group: many meshes assembled and solved as one block-diagonal system."""
+
+    def __init__(self, ctx, problems):
+        self.ctx, self.lib = ctx, ctx.lib
+        self.problems = list(problems)
+        keep = []
+        descs = (_lib.MeshDesc * len(self.problems))()
+        for k, p in enumerate(self.problems):
+            conn = np.ascontiguousarray(p.conn, dtype=np.int32)
+            coords = np.ascontiguousarray(p.coords, dtype=np.float64)
+            eE = None if p.elem_E is None else np.ascontiguousarray(p.elem_E, dtype=np.float64)
+            keep += [conn, coords, eE]
+            descs[k] = _lib.MeshDesc(p.elem_type, p.ngauss, coords.shape[0], conn.shape[0],
+                                     _ptr(conn), _ptr(coords), _ptr(eE), p.E, p.nu)
+        h = C.c_void_p()
+        ctx._check(self.lib.gf_group_create(ctx.handle, len(self.problems), descs, C.byref(h)))
+        self.handle = h
+        n, nnz, nc = C.c_int64(), C.c_int64(), C.c_int32()
+        check(self.lib.gf_group_info(h, C.byref(n), C.byref(nnz), C.byref(nc)))
+        self.n, self.nnz, self.ncolors = n.value, nnz.value, nc.value
+        self.offsets = np.cumsum([0] + [p.ndof for p in self.problems])
+
+    def assemble(self):
+        t = _lib.Timing()
+        self.ctx._check(self.lib.gf_group_assemble(self.ctx.handle, self.handle, C.byref(t)))
+        return t
+
+    def apply_penalty(self, coef=1e4):
+        dofs = np.ascontiguousarray(np.concatenate([p.fixed_dofs for p in self.problems]),
+                                    dtype=np.int32)
+        nd = np.ascontiguousarray([p.fixed_dofs.size for p in self.problems], dtype=np.int64)
+        self.ctx._check(self.lib.gf_group_apply_penalty(self.ctx.handle, self.handle, _ptr(dofs),
+                                                        _ptr(nd), coef))
+
+    def pcg(self, b=None, raise_on_fail=True, **opts):
+        if b is None:
+            b = np.concatenate([p.f for p in self.problems])
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b)
+        o = solve_opts(**opts)
+        st = _lib.SolveStats()
+        its = np.zeros(len(self.problems), dtype=np.int32)
+        code = self.lib.gf_group_pcg(self.ctx.handle, self.handle, _ptr(b), _ptr(x), C.byref(o),
+                                     C.byref(st), _ptr(its))
+        if raise_on_fail:
+            self.ctx._check(code)
+        st.code = code
+        return x, st, its
+
+    def split(self, x):
+        return [x[self.offsets[k]:self.offsets[k + 1]] for k in range(len(self.problems))]
+
+    def free(self):
+        if self.handle:
+            self.lib.gf_group_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DevBuf:
+    """Device allocation owned by a Context (gf_dev_alloc / gf_dev_free)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx, self.lib = ctx, ctx.lib
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        ctx._check(self.lib.gf_dev_alloc(ctx.handle, max(self.nbytes, 4), C.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_array(cls, ctx, a):
+        a = np.ascontiguousarray(a)
+        d = cls(ctx, a.nbytes)
+        d.upload(a)
+        return d
+
+    def upload(self, a, offset=0):
+        a = np.ascontiguousarray(a)
+        assert offset + a.nbytes <= self.nbytes
+        dst = C.c_void_p(self.ptr.value + offset)
+        self.ctx._check(self.lib.gf_copy_h2d(self.ctx.handle, dst, _ptr(a), a.nbytes))
+
+    def download(self, dtype, count=None, offset=0):
+        dt = np.dtype(dtype)
+        count = (self.nbytes - offset) // dt.itemsize if count is None else int(count)
+        out = np.empty(count, dtype=dt)
+        src = C.c_void_p(self.ptr.value + offset)
+        self.ctx._check(self.lib.gf_copy_d2h(self.ctx.handle, _ptr(out), src, out.nbytes))
+        return out
+
+    def at(self, offset):
+        """Device address `offset` bytes into the buffer."""
+        return C.c_void_p(self.ptr.value + int(offset))
+
+    def free(self):
+        if self.ptr:
+            self.lib.gf_dev_free(self.ctx.handle, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Net:
+    """gf_nn: the warm-start network on a node grid (nx, ny, nz)."""
+
+    def __init__(self, ctx, nx, ny, nz, batch_max, seed=0x5EED0007):
+        self.ctx, self.lib = ctx, ctx.lib
+        self.nx, self.ny, self.nz, self.batch_max = nx, ny, nz, batch_max
+        h = C.c_void_p()
+        ctx._check(self.lib.gf_nn_create(ctx.handle, nx, ny, nz, batch_max, C.c_uint64(seed),
+                                         C.byref(h)))
+        self.handle = h
+        self.nparams = self.lib.gf_nn_nparams(h)
+        self.nodes = nx * ny * nz
+
+    def layers(self):
+        out = []
+        info = np.zeros(8, dtype=np.int32)
+        for i in range(self.lib.gf_nn_layer_count(self.handle)):
+            check(self.lib.gf_nn_layer_info(self.handle, i, _ptr(info)))
+            out.append(tuple(int(v) for v in info))
+        return out
+
+    def params(self):
+        p = np.empty(self.nparams, dtype=np.float32)
+        self.ctx._check(self.lib.gf_nn_params_get(self.ctx.handle, self.handle, _ptr(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparams
+        self.ctx._check(self.lib.gf_nn_params_set(self.ctx.handle, self.handle, _ptr(p)))
+
+    def grads(self):
+        g = np.empty(self.nparams, dtype=np.float32)
+        self.ctx._check(self.lib.gf_nn_grads_get(self.ctx.handle, self.handle, _ptr(g)))
+        return g
+
+    def forward(self, in_dev, batch, out_dev=None):
+        self.ctx._check(self.lib.gf_nn_forward(self.ctx.handle, self.handle, in_dev, batch,
+                                               out_dev))
+
+    def backward(self, in_dev, target_dev, batch):
+        loss = C.c_float()
+        self.ctx._check(self.lib.gf_nn_backward(self.ctx.handle, self.handle, in_dev, target_dev,
+                                                batch, C.byref(loss)))
+        return loss.value
+
+    def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.ctx._check(self.lib.gf_nn_adam(self.ctx.handle, self.handle, lr, beta1, beta2, eps))
+
+    def train_step(self, in_dev, target_dev, batch, lr, want_loss=True):
+        loss = C.c_float()
+        self.ctx._check(self.lib.gf_nn_train_step(self.ctx.handle, self.handle, in_dev,
+                                                  target_dev, batch, lr,
+                                                  C.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
+
+    def to_dofs(self, pred_dev, in_dev, sample, mask_channel, scale, x_dev):
+        self.ctx._check(self.lib.gf_nn_to_dofs(self.ctx.handle, self.handle, pred_dev, in_dev,
+                                               sample, mask_channel, float(scale), x_dev))
+
+    def free(self):
+        if self.handle:
+            self.lib.gf_nn_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
         except Exception:
             pass

@@ -50,7 +50,7 @@ class SolveOpts(C.Structure):
     _fields_ = [("precond", C.c_int32), ("precision", C.c_int32), ("schedule", C.c_int32),
                 ("use_x0", C.c_int32), ("maxit", C.c_int32), ("device_ptrs", C.c_int32),
                 ("rtol", C.c_double), ("inner_rtol", C.c_double), ("batched", C.c_int32),
-                ("pad_", C.c_int32)]
+                ("verify", C.c_int32)]
 
 
 class SolveStats(C.Structure):
@@ -71,12 +71,14 @@ SIGNATURES = {
     "gf_grid_face_load": (C.c_int, [C.POINTER(GridDesc), _I32, _I32, _D, _P]),
     "gf_rng_normal": (C.c_int, [C.c_uint64, _I64, _P]),
     "gf_rng_uniform_int": (C.c_int, [C.c_uint64, _I64, _I32, _I32, _P]),
+    "gf_grf": (C.c_int, [C.c_uint64, _I64, _I64, _I64, _D, _P]),
     "gf_create": (C.c_int, [C.POINTER(_P), _I32]),
     "gf_destroy": (C.c_int, [_P]),
     "gf_last_error": (C.c_char_p, [_P]),
     "gf_device_info": (C.c_int, [_P, _P, C.c_char_p, _I32]),
     "gf_mesh_upload": (C.c_int, [_P, C.POINTER(MeshDesc), C.POINTER(_P)]),
     "gf_mesh_free": (C.c_int, [_P]),
+    "gf_mesh_set_E": (C.c_int, [_P, _P, _P]),
     "gf_ell_symbolic": (C.c_int, [_P, _P, C.POINTER(_P), C.POINTER(Timing)]),
     "gf_ell_info_get": (C.c_int, [_P, C.POINTER(EllInfo)]),
     "gf_ell_colors": (C.c_int, [_P, _P, _P]),
@@ -90,10 +92,33 @@ SIGNATURES = {
     "gf_pcg": (C.c_int, [_P, _P, _P, _P, _I32, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "gf_ell_tune": (C.c_int, [_P, _P, _I32, _P, _P]),
     "gf_profile_spmv": (C.c_int, [_P, _P, _I32, C.POINTER(C.c_double)]),
+    "gf_group_create": (C.c_int, [_P, _I32, C.POINTER(MeshDesc), C.POINTER(_P)]),
+    "gf_group_info": (C.c_int, [_P, _P, _P, _P]),
+    "gf_group_assemble": (C.c_int, [_P, _P, C.POINTER(Timing)]),
+    "gf_group_apply_penalty": (C.c_int, [_P, _P, _P, _P, _D]),
+    "gf_group_pcg": (C.c_int, [_P, _P, _P, _P, C.POINTER(SolveOpts), C.POINTER(SolveStats), _P]),
+    "gf_group_free": (C.c_int, [_P]),
+    "gf_nn_create": (C.c_int, [_P, _I32, _I32, _I32, _I32, C.c_uint64, C.POINTER(_P)]),
+    "gf_nn_free": (C.c_int, [_P]),
+    "gf_nn_nparams": (C.c_int64, [_P]),
+    "gf_nn_layer_count": (C.c_int, [_P]),
+    "gf_nn_layer_info": (C.c_int, [_P, _I32, _P]),
+    "gf_nn_params_get": (C.c_int, [_P, _P, _P]),
+    "gf_nn_params_set": (C.c_int, [_P, _P, _P]),
+    "gf_nn_grads_get": (C.c_int, [_P, _P, _P]),
+    "gf_nn_forward": (C.c_int, [_P, _P, _P, _I32, _P]),
+    "gf_nn_backward": (C.c_int, [_P, _P, _P, _P, _I32, _P]),
+    "gf_nn_adam": (C.c_int, [_P, _P, C.c_float, C.c_float, C.c_float, C.c_float]),
+    "gf_nn_train_step": (C.c_int, [_P, _P, _P, _P, _I32, C.c_float, _P]),
+    "gf_nn_to_dofs": (C.c_int, [_P, _P, _P, _P, _I32, _I32, _D, _P]),
+    "gf_nn_conv3d": (C.c_int, [_P, _P, _I32, _P, _I32, _P, _P, _I32, _I32, _I32, _I32,
+                               C.c_float, _P, _P, _P, _I32]),
+    "gf_nn_conv3d_wgrad": (C.c_int, [_P, _P, _I32, _P, _I32, _P, _P, _I32, _I32, _P, _P]),
     "gf_dev_alloc": (C.c_int, [_P, _I64, C.POINTER(_P)]),
     "gf_dev_free": (C.c_int, [_P, _P]),
     "gf_copy_h2d": (C.c_int, [_P, _P, _P, _I64]),
     "gf_copy_d2h": (C.c_int, [_P, _P, _P, _I64]),
+    "gf_copy_d2d": (C.c_int, [_P, _P, _P, _I64]),
 }
 
 _lib = None

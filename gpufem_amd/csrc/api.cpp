@@ -193,6 +193,17 @@ int gf_mesh_upload(gf_ctx* ctx, const gf_mesh_desc* d, gf_mesh** out) {
   return GF_OK;
 }
 
+int gf_mesh_set_E(gf_ctx* ctx, gf_mesh* m, const double* elem_E) {
+  if (!ctx || !m || !elem_E) return set_error(ctx, GF_ERR_ARG, "gf_mesh_set_E: null argument");
+  int st = select_device(ctx);
+  if (st) return st;
+  const size_t eb = size_t(m->nelem) * sizeof(double);
+  if (!m->d_E) GF_HIP_CHECK(ctx, hipMalloc(&m->d_E, eb));
+  GF_HIP_CHECK(ctx, hipMemcpyAsync(m->d_E, elem_E, eb, hipMemcpyHostToDevice, ctx->stream));
+  GF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return GF_OK;
+}
+
 int gf_mesh_free(gf_mesh* m) {
   if (!m) return GF_OK;
   (void)hipSetDevice(m->dev);
@@ -522,6 +533,13 @@ int gf_dev_alloc(gf_ctx* ctx, int64_t bytes, void** dptr) {
 int gf_dev_free(gf_ctx* ctx, void* dptr) {
   if (!ctx) return set_error(ctx, GF_ERR_ARG, "gf_dev_free: null context");
   if (dptr) GF_HIP_CHECK(ctx, hipFree(dptr));
+  return GF_OK;
+}
+
+int gf_copy_d2d(gf_ctx* ctx, void* dst, const void* src, int64_t bytes) {
+  if (!ctx || bytes < 0 || (bytes && (!dst || !src)))
+    return set_error(ctx, GF_ERR_ARG, "gf_copy_d2d: bad argument");
+  GF_HIP_CHECK(ctx, hipMemcpyAsync(dst, src, size_t(bytes), hipMemcpyDeviceToDevice, ctx->stream));
   return GF_OK;
 }
 

@@ -1,0 +1,49 @@
+// Grouped batch of independent meshes (kernels in group.hip, host side in
+// group.cpp).
+#pragma once
+
+#include "gf_internal.hpp"
+#include "launch.hpp"
+
+namespace gf {
+
+struct GroupState {
+  int32_t it, done, maxit, pad;
+  double rtol2;
+};
+
+struct GroupBuffers {
+  int64_t n;
+  int nmesh;
+  int64_t ntiles;
+  int nblocks;
+  int variant;
+  const int64_t* off;        // [nmesh+1] first row of each mesh
+  const int64_t* tile_ptr;   // [nmesh+1] first tile of each mesh
+  const int32_t* tile_mesh;  // [ntiles]
+  const int64_t* tile_r0;    // [ntiles]
+  const int64_t* tile_r1;    // [ntiles]
+  double* val;
+  const int32_t* col;
+  const int32_t* rownnz;
+  const int32_t* diag;
+  const double* dinv;
+  const double* b;
+  double *x, *r, *z, *p, *q;
+  double* part;  // [3 * ntiles]
+  double *rho, *alpha, *beta, *rr, *bb, *pen;  // [nmesh]
+  int32_t *it, *done, *status;                 // [nmesh]
+  GroupState* st;
+};
+
+size_t group_state_bytes();
+int group_blocks(int64_t ntiles);
+hipError_t launch_group_penalty(hipStream_t s, const GroupBuffers& G, double coef,
+                                const int64_t* dofs, const int32_t* mesh, const int32_t* counts,
+                                int64_t nd);
+hipError_t launch_group_setup(hipStream_t s, const GroupBuffers& G, int use_x0, double rtol,
+                              int maxit);
+hipError_t launch_group_iteration(hipStream_t s, const GroupBuffers& G);
+hipError_t launch_group_residual(hipStream_t s, const GroupBuffers& G, double* out);
+
+}  // namespace gf

@@ -205,6 +205,62 @@ int gf_rng_normal(uint64_t seed, int64_t n, double* out) {
   return GF_OK;
 }
 
+int gf_grf(uint64_t seed, int64_t nx, int64_t ny, int64_t nz, double lengthscale,
+           double* out) {
+  if (nx < 1 || ny < 1 || nz < 1 || !(lengthscale > 0.0) || !out) return GF_ERR_ARG;
+  // white noise on the grid extended by R cells per side along every axis
+  // of extent > 1, smoothed by a separable Gaussian (std = lengthscale)
+  // with per-axis weights scaled to unit sum of squares: every output cell
+  // is a unit-variance combination of the same number of noise cells
+  const int64_t R = int64_t(std::ceil(3.0 * lengthscale));
+  const int64_t n_[3] = {nx, ny, nz};
+  int64_t e_[3], r_[3];
+  for (int a = 0; a < 3; ++a) {
+    r_[a] = n_[a] > 1 ? R : 0;
+    e_[a] = n_[a] + 2 * r_[a];
+  }
+  std::vector<double> w(size_t(2 * R + 1));
+  double s2 = 0.0;
+  for (int64_t k = -R; k <= R; ++k) {
+    const double v = std::exp(-0.5 * double(k * k) / (lengthscale * lengthscale));
+    w[size_t(k + R)] = v;
+    s2 += v * v;
+  }
+  for (double& v : w) v /= std::sqrt(s2);
+  std::vector<double> a(static_cast<size_t>(e_[0] * e_[1] * e_[2]));
+  gf_rng_normal(seed, int64_t(a.size()), a.data());
+  // pass along x: (e0, e1, e2) -> (n0, e1, e2)
+  std::vector<double> b(static_cast<size_t>(nx * e_[1] * e_[2]), 0.0);
+  for (int64_t k = 0; k < e_[2]; ++k)
+    for (int64_t j = 0; j < e_[1]; ++j)
+      for (int64_t i = 0; i < nx; ++i) {
+        double s = 0.0;
+        for (int64_t t = -r_[0]; t <= r_[0]; ++t)
+          s += w[size_t(t + R)] * a[size_t((k * e_[1] + j) * e_[0] + i + r_[0] + t)];
+        b[size_t((k * e_[1] + j) * nx + i)] = r_[0] ? s : a[size_t((k * e_[1] + j) * e_[0] + i)];
+      }
+  // along y: -> (n0, n1, e2)
+  std::vector<double> c(static_cast<size_t>(nx * ny * e_[2]), 0.0);
+  for (int64_t k = 0; k < e_[2]; ++k)
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t i = 0; i < nx; ++i) {
+        double s = 0.0;
+        for (int64_t t = -r_[1]; t <= r_[1]; ++t)
+          s += w[size_t(t + R)] * b[size_t((k * e_[1] + j + r_[1] + t) * nx + i)];
+        c[size_t((k * ny + j) * nx + i)] = r_[1] ? s : b[size_t((k * e_[1] + j) * nx + i)];
+      }
+  // along z: -> (n0, n1, n2)
+  for (int64_t k = 0; k < nz; ++k)
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t i = 0; i < nx; ++i) {
+        double s = 0.0;
+        for (int64_t t = -r_[2]; t <= r_[2]; ++t)
+          s += w[size_t(t + R)] * c[size_t(((k + r_[2] + t) * ny + j) * nx + i)];
+        out[size_t((k * ny + j) * nx + i)] = r_[2] ? s : c[size_t((k * ny + j) * nx + i)];
+      }
+  return GF_OK;
+}
+
 int gf_rng_uniform_int(uint64_t seed, int64_t n, int32_t lo, int32_t hi, int32_t* out) {
   if (n < 0 || hi < lo || (n > 0 && !out)) return GF_ERR_ARG;
   SplitMix64 rng(seed);

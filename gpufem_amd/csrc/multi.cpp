@@ -1,5 +1,7 @@
 // Host driver of the batched PCG (kernels in multi.hip).
+#include <algorithm>
 #include <string>
+#include <vector>
 
 #include "multi.hpp"
 #include "solve_plan.hpp"
@@ -18,6 +20,7 @@ int set_error(gf_ctx* ctx, int code, const std::string& msg);
 
 namespace {
 constexpr int kChunk = 16;
+constexpr int kMaxRestarts = 4;  // true-residual restarts
 }
 
 void release_multi(gf_solve_plan* P) {
@@ -80,17 +83,37 @@ int run_multi(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, const double* B
   B.st = W->st;
   GF_HIP_CHECK(ctx, launch_multi_pack(ctx->stream, S, n, ncols, Bd, W->b));
   if (opts.use_x0) GF_HIP_CHECK(ctx, launch_multi_pack(ctx->stream, S, n, ncols, Xd, W->x));
-  GF_HIP_CHECK(ctx, launch_multi_setup(ctx->stream, B, opts.rtol, opts.maxit));
-  GF_HIP_CHECK(ctx, read_multi_state(ctx->stream, B, h));
-  while (!h->done) {
-    for (int it = 0; it < kChunk; ++it) GF_HIP_CHECK(ctx, launch_multi_iteration(ctx->stream, B));
+  std::vector<int32_t> col_total(static_cast<size_t>(ncols), 0);
+  int total = 0;
+  for (int restart = 0;; ++restart) {
+    GF_HIP_CHECK(ctx, launch_multi_setup(ctx->stream, B, opts.rtol, opts.maxit - total));
     GF_HIP_CHECK(ctx, read_multi_state(ctx->stream, B, h));
+    while (!h->done) {
+      for (int it = 0; it < kChunk; ++it) GF_HIP_CHECK(ctx, launch_multi_iteration(ctx->stream, B));
+      GF_HIP_CHECK(ctx, read_multi_state(ctx->stream, B, h));
+    }
+    for (int k = 0; k < ncols; ++k) {
+      col_total[size_t(k)] += h->col_it[k];
+      h->col_it[k] = col_total[size_t(k)];
+    }
+    total = *std::max_element(col_total.begin(), col_total.end());
+    GF_HIP_CHECK(ctx, launch_multi_residual(ctx->stream, B, W->res));
+    GF_HIP_CHECK(ctx, hipMemcpyAsync(true_res, W->res, size_t(ncols) * sizeof(double),
+                                     hipMemcpyDeviceToHost, ctx->stream));
+    GF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    // restart (from the current X) when some column's recursive residual met
+    // rtol while b - K x did not; the other columns are done at the restart's
+    // setup.  A breakdown is reported as it is.
+    bool again = false, healthy = true;
+    for (int k = 0; k < ncols; ++k) {
+      healthy &= h->col_status[k] == GF_OK;
+      again |= h->rr[k] <= opts.rtol * opts.rtol * h->bnorm2[k] && true_res[k] > opts.rtol;
+    }
+    again &= healthy;
+    if (!opts.verify || !again || restart >= kMaxRestarts || total >= opts.maxit) break;
+    B.use_x0 = 1;
   }
-  GF_HIP_CHECK(ctx, launch_multi_residual(ctx->stream, B, W->res));
   GF_HIP_CHECK(ctx, launch_multi_unpack(ctx->stream, S, n, ncols, W->x, Xd));
-  GF_HIP_CHECK(ctx, hipMemcpyAsync(true_res, W->res, size_t(ncols) * sizeof(double),
-                                   hipMemcpyDeviceToHost, ctx->stream));
-  GF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return GF_OK;
 }
 

@@ -2,7 +2,9 @@
 //
 // Semantics follow SPRmatrix::CPU_CG (src/SPRmatrix/SPRmatrix.cc:405-480)
 // extended with the Jacobi preconditioner: x0 = 0 (or the caller's guess),
-// stop when ||r||_2 <= rtol ||b||_2, report the true residual at exit.  The
+// stop when ||r||_2 <= rtol ||b||_2 (recursive r); with opts.verify the true
+// residual b - K x is then checked and PCG restarted from x while it exceeds
+// rtol.  The true residual at exit is always reported.  The
 // workspace (vectors, partials, scalar state, captured graphs) is cached on
 // the matrix; the SpMV variant is picked once per matrix by timing every
 // variant on the matrix itself.
@@ -26,6 +28,7 @@ int set_error(gf_ctx* ctx, int code, const std::string& msg);
 namespace {
 
 constexpr int kChunk = 32;  // iterations between convergence checks (even)
+constexpr int kMaxRestarts = 4;  // true-residual restarts per right-hand side
 
 double now_ms() {
   using namespace std::chrono;
@@ -160,27 +163,43 @@ int tune_spmv(gf_ctx* ctx, gf_ell* m, int force, double* ms_out, int* chosen) {
 // Runs one right-hand side already placed in the plan's wb (and wx when
 // use_x0) to completion with the given schedule.
 int run_pcg(gf_ctx* ctx, gf_ell* m, const PcgBuffers& B, int sched, int key, int use_x0,
-            double rtol, int maxit, PcgHostState* h) {
+            double rtol, int maxit, int verify, PcgHostState* h) {
   gf_solve_plan* P = m->plan;
-  GF_HIP_CHECK(ctx, launch_pcg_setup(ctx->stream, B, use_x0, rtol, maxit));
-  if (sched == GF_SCHED_PERSISTENT) {
-    GF_HIP_CHECK(ctx, launch_pcg_persistent(ctx->stream, B, P->barrier));
-    GF_HIP_CHECK(ctx, read_pcg_state(ctx->stream, B, h));
-    if (h->status == GF_ERR_HIP)
-      return set_error(ctx, GF_ERR_HIP, "gf_pcg: grid barrier timed out in the persistent kernel");
-    return GF_OK;
-  }
-  GF_HIP_CHECK(ctx, read_pcg_state(ctx->stream, B, h));
-  while (!h->done) {
-    if (sched == GF_SCHED_GRAPH) {
-      GF_HIP_CHECK(ctx, ensure_graph(ctx, m, B, B.dinv ? 1 : 0));
-      GF_HIP_CHECK(ctx, hipGraphLaunch(P->graph[key], ctx->stream));
+  int total = 0;
+  for (int restart = 0;; ++restart) {
+    GF_HIP_CHECK(ctx, launch_pcg_setup(ctx->stream, B, use_x0, rtol, maxit - total));
+    if (sched == GF_SCHED_PERSISTENT) {
+      GF_HIP_CHECK(ctx, launch_pcg_persistent(ctx->stream, B, P->barrier));
+      GF_HIP_CHECK(ctx, read_pcg_state(ctx->stream, B, h));
+      if (h->status == GF_ERR_HIP)
+        return set_error(ctx, GF_ERR_HIP, "gf_pcg: grid barrier timed out in the persistent kernel");
     } else {
-      for (int it = 0; it < kChunk; ++it) GF_HIP_CHECK(ctx, launch_pcg_iteration(ctx->stream, B, it & 1));
+      GF_HIP_CHECK(ctx, read_pcg_state(ctx->stream, B, h));
+      while (!h->done) {
+        if (sched == GF_SCHED_GRAPH) {
+          GF_HIP_CHECK(ctx, ensure_graph(ctx, m, B, B.dinv ? 1 : 0));
+          GF_HIP_CHECK(ctx, hipGraphLaunch(P->graph[key], ctx->stream));
+        } else {
+          for (int it = 0; it < kChunk; ++it)
+            GF_HIP_CHECK(ctx, launch_pcg_iteration(ctx->stream, B, it & 1));
+        }
+        GF_HIP_CHECK(ctx, read_pcg_state(ctx->stream, B, h));
+      }
     }
-    GF_HIP_CHECK(ctx, read_pcg_state(ctx->stream, B, h));
+    total += h->it;
+    h->it = total;
+    // The recursive residual drifts from b - K x by rounding accumulated over
+    // many iterations; a restart from the current x resets the drift.
+    if (!verify || restart >= kMaxRestarts || h->status != GF_OK || total >= maxit ||
+        !(h->rr <= rtol * rtol * h->bnorm2))
+      return GF_OK;
+    double tr = 0.0;
+    GF_HIP_CHECK(ctx, launch_true_residual(ctx->stream, B, P->res));
+    GF_HIP_CHECK(ctx, hipMemcpyAsync(&tr, P->res, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    GF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (tr <= rtol) return GF_OK;
+    use_x0 = 1;
   }
-  return GF_OK;
 }
 
 int prepare_jacobi(gf_ctx* ctx, gf_ell* m) {
@@ -224,6 +243,7 @@ void gf_solve_opts_default(gf_solve_opts* o) {
   o->rtol = 1e-8;
   o->inner_rtol = 1e-4;
   o->batched = 1;
+  o->verify = 1;
 }
 
 int gf_ell_tune(gf_ctx* ctx, gf_ell* m, int32_t force_variant, double* ms_per_variant,
@@ -343,13 +363,13 @@ int gf_pcg(gf_ctx* ctx, gf_ell* m, const double* b, double* x, int32_t nrhs,
         const int mixed_it = h.it;
         PcgBuffers Bj = make_buffers(m, m->d_dinv);
         st = run_pcg(ctx, m, Bj, sched == GF_SCHED_PERSISTENT ? sched : GF_SCHED_KERNELS, 1, 0,
-                     opts.rtol, opts.maxit, &h);
+                     opts.rtol, opts.maxit, opts.verify, &h);
         if (st) return st;
         h.it += mixed_it;
         ++fallbacks;
       }
     } else {
-      st = run_pcg(ctx, m, B, sched, key, opts.use_x0, opts.rtol, opts.maxit, &h);
+      st = run_pcg(ctx, m, B, sched, key, opts.use_x0, opts.rtol, opts.maxit, opts.verify, &h);
       if (st) return st;
     }
     GF_HIP_CHECK(ctx, launch_true_residual(ctx->stream, B, P->res + k));

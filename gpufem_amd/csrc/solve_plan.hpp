@@ -34,5 +34,5 @@ int tune_spmv(gf_ctx* ctx, gf_ell* m, int force, double* ms_out, int* chosen);
 PcgBuffers make_buffers(gf_ell* m, const double* dinv);
 int prepare_jacobi(gf_ctx* ctx, gf_ell* m);
 int run_pcg(gf_ctx* ctx, gf_ell* m, const PcgBuffers& B, int sched, int key, int use_x0,
-            double rtol, int maxit, PcgHostState* h);
+            double rtol, int maxit, int verify, PcgHostState* h);
 }  // namespace gf

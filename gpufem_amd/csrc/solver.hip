@@ -39,8 +39,7 @@ struct PcgState {
 };
 
 struct GridBarrier {
-  unsigned count;
-  unsigned gen;
+  unsigned long long count;  // arrivals since the launch began (monotonic)
   int abort;
   int pad;
 };
@@ -204,30 +203,30 @@ __global__ __launch_bounds__(kNT) void k_direction(PcgBuffers B, int parity) {
 // ---------------------------------------------------------------------------
 // bounded grid barrier for the cooperative kernel
 // ---------------------------------------------------------------------------
-constexpr long kSpinLimit = 1L << 24;  // ~1 s of polling: a barrier never takes that long
+constexpr long kSpinLimit = 1L << 22;  // several seconds of polling: a barrier never takes that long
 
+// Sense-free barrier on a monotonic arrival counter: arrival number a (0-based)
+// belongs to barrier a / nb and waits until the counter reaches the end of
+// that barrier's range.  The poll is a relaxed load (no cache maintenance per
+// poll: an acquire load would invalidate the XCD's L2 on every spin, under the
+// feet of the waves still gathering through it); one acquire fence follows.
 __device__ inline bool grid_barrier(GridBarrier* bar, int* s_flag) {
   // the workgroup barrier makes every wave's stores complete; one agent-scope
   // release (L2 write-back) then publishes them for the other XCDs
   __syncthreads();
   if (threadIdx.x == 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    const unsigned nb = gridDim.x;
-    const unsigned g = __hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned arrived =
-        __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (arrived == nb - 1) {
-      __hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&bar->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      long spins = 0;
-      while (__hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
-        __builtin_amdgcn_s_sleep(2);
-        if (__hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-        if (++spins > kSpinLimit) {
-          __hip_atomic_store(&bar->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
+    const unsigned long long nb = gridDim.x;
+    const unsigned long long arrived =
+        __hip_atomic_fetch_add(&bar->count, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long target = (arrived / nb + 1) * nb;
+    long spins = 0;
+    while (__hip_atomic_load(&bar->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(1);
+      if (__hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+      if (++spins > kSpinLimit) {
+        __hip_atomic_store(&bar->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
       }
     }
     *s_flag = __hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -299,7 +298,6 @@ __global__ __launch_bounds__(kNT) void k_pcg_persistent(PcgBuffers B, GridBarrie
 __global__ void k_barrier_reset(GridBarrier* bar) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     bar->count = 0;
-    bar->gen = 0;
     bar->abort = 0;
   }
 }

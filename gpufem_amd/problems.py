@@ -135,3 +135,66 @@ def config_brick20_cube(n=48):
 def q4_batch_sizes(count=256, lo=16, hi=256, seed=SEED_Q4_BATCH):
     nx = rng_uniform_int(seed, count, lo, hi)
     return [(int(a), int(a) // 2) for a in nx]
+
+
+# ---- learned warm-start family: Brick8 cantilevers ------------------------------
+SEED_CANTILEVER = 0x5EED0006
+CANTILEVER_LENGTHSCALE = 8.0
+CANTILEVER_SIGMA = 0.5
+CANTILEVER_TRACTION_MEAN = (0.0, 0.0, -1.0)
+CANTILEVER_TRACTION_STD = 0.5
+
+
+def cantilever_seeds(k):
+    """Seeds of member k: (E field, traction x, traction y, traction z)."""
+    base = SEED_CANTILEVER + 4 * int(k)
+    return base, base + 1, base + 2, base + 3
+
+
+def cantilever_fields(k, nx=64, ny=32, nz=32, lengthscale=CANTILEVER_LENGTHSCALE,
+                      sigma=CANTILEVER_SIGMA):
+    """Element moduli and top-face tractions of member k.
+
+    E_e = 210000 exp(sigma g_e - sigma^2 / 2), g a unit-variance Gaussian random
+    field over the elements (mean of E is 210000); traction component c at
+    top-face node (i, j) is mu_c + s g_c(i, j), g_c a unit-variance field over
+    the (nx+1) x (ny+1) face nodes.  Returns (elem_E [nel], traction [ny+1, nx+1, 3]).
+    """
+    from . import grf
+    sE, sx, sy, sz = cantilever_seeds(k)
+    g = grf(sE, (nx, ny, nz), lengthscale).reshape(-1)
+    elem_E = E_DEFAULT * np.exp(sigma * g - 0.5 * sigma * sigma)
+    t = np.empty((ny + 1, nx + 1, 3))
+    for c, s in enumerate((sx, sy, sz)):
+        t[:, :, c] = (CANTILEVER_TRACTION_MEAN[c] +
+                      CANTILEVER_TRACTION_STD * grf(s, (nx + 1, ny + 1), lengthscale))
+    return elem_E, t
+
+
+def cantilever_problem(k, nx=64, ny=32, nz=32, template=None):
+    """Member k of the Brick8 cantilever family (2x2x2 Gauss, nu = 0.3).
+
+    The x = 0 face is clamped; the top face (z = nz) carries the random
+    traction of cantilever_fields lumped to its nodes (nodal force = traction
+    times the node's tributary area: 1/4 per adjacent unit face).  `template`
+    (a member of the same size) lends its mesh arrays.
+    """
+    elem_E, t = cantilever_fields(k, nx, ny, nz)
+    if template is None:
+        g = Grid(GF_BRICK8, nx, ny, nz)
+        conn, coords = g.fill()
+        sup = _supports_from_nodes(g.face_nodes(0), 3)
+        fixed = _fixed_dofs(sup, 3)
+    else:
+        g, conn, coords = template.grid, template.conn, template.coords
+        sup, fixed = template.supports, template.fixed_dofs
+    area = np.ones((ny + 1, nx + 1))
+    area[0, :] *= 0.5
+    area[-1, :] *= 0.5
+    area[:, 0] *= 0.5
+    area[:, -1] *= 0.5
+    F = np.zeros((nz + 1, ny + 1, nx + 1, 3))
+    F[nz] = t * area[:, :, None]
+    f = F.reshape(-1)
+    return Problem("brick8_cantilever_%dx%dx%d_k%d" % (nx, ny, nz, k), GF_BRICK8, 2, g, conn,
+                   coords, E_DEFAULT, NU_DEFAULT, elem_E, sup, fixed, _loads_from_f(f, 3), f)

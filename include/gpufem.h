@@ -93,6 +93,13 @@ int gf_grid_face_load(const gf_grid_desc* g, int32_t face, int32_t comp, double 
 /* Deterministic streams (SplitMix64 + Box-Muller). */
 int gf_rng_normal(uint64_t seed, int64_t n, double* out);
 int gf_rng_uniform_int(uint64_t seed, int64_t n, int32_t lo, int32_t hi, int32_t* out);
+/* Unit-variance Gaussian random field on an nx x ny x nz cell grid (x
+ * fastest; nz = 1 for 2D): white noise from gf_rng_normal(seed) on the grid
+ * extended by ceil(3 lengthscale) cells per side of every axis longer than 1,
+ * smoothed by a separable Gaussian of standard deviation `lengthscale` cells
+ * whose per-axis weights have unit sum of squares.  No reference counterpart
+ * (random modulus and load fields of the learned warm-start family). */
+int gf_grf(uint64_t seed, int64_t nx, int64_t ny, int64_t nz, double lengthscale, double* out);
 
 /* ======================================================================== */
 /* Context                                                                   */
@@ -120,6 +127,10 @@ typedef struct gf_mesh_desc {
 
 int gf_mesh_upload(gf_ctx* ctx, const gf_mesh_desc* desc, gf_mesh** out);
 int gf_mesh_free(gf_mesh* mesh);
+/* Replaces the per-element moduli (nelem doubles, host) of an uploaded mesh;
+ * the next gf_assemble uses them.  Stands in for re-reading the material
+ * table of FemData (src/FEM/FemData.cc:322-365) without a new upload. */
+int gf_mesh_set_E(gf_ctx* ctx, gf_mesh* mesh, const double* elem_E);
 
 /* ======================================================================== */
 /* ELL matrix.  Slot-major ("column-wise") ELLPACK as the reference's       */
@@ -190,7 +201,10 @@ typedef struct gf_solve_opts {
   double inner_rtol; /* reserved */
   int32_t batched;   /* nrhs > 1 (FP64): 1 = all right-hand sides advance together
                         with one ELL SpMM per iteration; 0 = one after another */
-  int32_t pad_;
+  int32_t verify;    /* 1 (default): when the recursive residual meets rtol, recompute
+                        ||b - K x|| in FP64 and, while it exceeds rtol, restart PCG from
+                        the current x (at most 4 times); 0: stop on the recursive
+                        residual alone, as the reference CG does */
 } gf_solve_opts;
 
 typedef struct gf_solve_stats {
@@ -231,11 +245,87 @@ int gf_ell_tune(gf_ctx* ctx, gf_ell* m, int32_t force_variant, double* ms_per_va
  * (src/OpenCL/OCLwrapper.cc:400-434) for the roofline figure. */
 int gf_profile_spmv(gf_ctx* ctx, gf_ell* m, int32_t nrep, double* ms_avg);
 
+/* ======================================================================== */
+/* Grouped batch: many independent meshes of one element type (same Gauss   */
+/* rule and nu; E may differ) assembled and solved together as one          */
+/* block-diagonal system.  Assembly is one launch per color over all the    */
+/* meshes; the penalty factor, the PCG scalars and the stopping test are    */
+/* per mesh.  No reference counterpart: the reference runs its batches one  */
+/* file after another (src/interface.cpp:503-605).                          */
+/* ======================================================================== */
+typedef struct gf_group gf_group;
+int gf_group_create(gf_ctx* ctx, int32_t nmesh, const gf_mesh_desc* meshes, gf_group** out);
+int gf_group_info(const gf_group* g, int64_t* total_dof, int64_t* nnz, int32_t* ncolors);
+int gf_group_assemble(gf_ctx* ctx, gf_group* g, gf_timing* t);
+/* dofs: fixed DOFs of every mesh (0-based within its mesh) concatenated;
+ * ndofs[m] of them belong to mesh m.  pen_m = coef * max diag of mesh m. */
+int gf_group_apply_penalty(gf_ctx* ctx, gf_group* g, const int32_t* dofs, const int64_t* ndofs,
+                           double coef);
+/* b, x: all meshes concatenated (mesh m after meshes 0..m-1).  Jacobi-PCG,
+ * FP64.  iterations_per_mesh (may be NULL): [nmesh]. */
+int gf_group_pcg(gf_ctx* ctx, gf_group* g, const double* b, double* x, const gf_solve_opts* opts,
+                 gf_solve_stats* stats, int32_t* iterations_per_mesh);
+int gf_group_free(gf_group* g);
+
+/* ======================================================================== */
+/* Learned warm start: a BF16 3D convolutional encoder-decoder on the node  */
+/* grid of a structured mesh, trained on displacement fields this solver    */
+/* computes, whose prediction is the PCG initial guess (opts.use_x0 = 1;    */
+/* zero-start PCG stays the fallback and every answer meets opts.rtol).     */
+/* No reference counterpart.  Tensors are device pointers, channels-last:  */
+/* input [batch][nz][ny][nx][16] bf16 bit patterns (features, zero-padded), */
+/* output and target [batch][nodes][16] / [batch][nodes][4] fp32 (the first */
+/* 3 channels are displacement).  Every node-grid axis is 1 mod 8.          */
+/* ======================================================================== */
+typedef struct gf_nn gf_nn;
+int gf_nn_create(gf_ctx* ctx, int32_t nx, int32_t ny, int32_t nz, int32_t batch_max,
+                 uint64_t seed, gf_nn** out);
+int gf_nn_free(gf_nn* net);
+int64_t gf_nn_nparams(const gf_nn* net);
+int gf_nn_layer_count(const gf_nn* net);
+/* info[8] = cin, cout, stride, activation, input level, output level,
+ * weight offset, bias offset (weights W[27][cin][cout], tap = 9(dz+1) +
+ * 3(dy+1) + (dx+1), in the flat fp32 parameter vector) */
+int gf_nn_layer_info(const gf_nn* net, int32_t layer, int32_t* info);
+int gf_nn_params_get(gf_ctx* ctx, gf_nn* net, float* host);
+/* sets the parameters and resets the Adam moments */
+int gf_nn_params_set(gf_ctx* ctx, gf_nn* net, const float* host);
+int gf_nn_grads_get(gf_ctx* ctx, gf_nn* net, float* host);
+/* out_dev (may be NULL): [batch][nodes][16] fp32 */
+int gf_nn_forward(gf_ctx* ctx, gf_nn* net, const uint16_t* in_dev, int32_t batch, float* out_dev);
+/* forward + mean squared error over the 3 displacement channels + backward:
+ * leaves the parameter gradient on the device (gf_nn_grads_get) */
+int gf_nn_backward(gf_ctx* ctx, gf_nn* net, const uint16_t* in_dev, const float* target_dev,
+                   int32_t batch, float* loss_host);
+int gf_nn_adam(gf_ctx* ctx, gf_nn* net, float lr, float beta1, float beta2, float eps);
+/* gf_nn_backward then Adam (0.9, 0.999, 1e-8); loss_host may be NULL */
+int gf_nn_train_step(gf_ctx* ctx, gf_nn* net, const uint16_t* in_dev, const float* target_dev,
+                     int32_t batch, float lr, float* loss_host);
+/* x_dev[3 i + c] = scale * pred[sample][i][c], zero at nodes whose input
+ * feature channel mask_channel exceeds 1/2; x_dev is an FP64 DOF vector */
+int gf_nn_to_dofs(gf_ctx* ctx, gf_nn* net, const float* pred_dev, const uint16_t* in_dev,
+                  int32_t sample, int32_t mask_channel, double scale, double* x_dev);
+/* One 3x3x3 convolution layer through the same kernels (zero padding):
+ *   transposed = 0: out[v] = act(bias + sum_t in[s v + t] . W[t])  W: [27][cin][cout]
+ *   transposed = 1: out[u] = sum_t in[(u - t)/s] . W[t]^T           W: [27][cout][cin]
+ * vin/vout: node grids {nx, ny, nz}; channels multiples of 16; exactly one
+ * of out_dev (bf16) / out32_dev (fp32) non-NULL; accumulate adds to it. */
+int gf_nn_conv3d(gf_ctx* ctx, const uint16_t* in_dev, int32_t batch, const int32_t* vin,
+                 int32_t cin, const float* w_dev, const float* bias_dev, int32_t cout,
+                 int32_t stride, int32_t transposed, int32_t act, float slope, const int32_t* vout,
+                 uint16_t* out_dev, float* out32_dev, int32_t accumulate);
+/* dW[t][ci][co] = sum_v x[s v + t][ci] dz[v][co]; db[co] = sum_v dz[v][co] (db may be NULL) */
+int gf_nn_conv3d_wgrad(gf_ctx* ctx, const uint16_t* x_dev, int32_t batch, const int32_t* vin,
+                       int32_t cin, const uint16_t* dz_dev, const int32_t* vout, int32_t cout,
+                       int32_t stride, float* dw_dev, float* db_dev);
+
 /* Device memory helpers for callers that keep vectors resident in HBM. */
 int gf_dev_alloc(gf_ctx* ctx, int64_t bytes, void** dptr);
 int gf_dev_free(gf_ctx* ctx, void* dptr);
 int gf_copy_h2d(gf_ctx* ctx, void* dst, const void* src, int64_t bytes);
 int gf_copy_d2h(gf_ctx* ctx, void* dst, const void* src, int64_t bytes);
+/* stream-ordered device-to-device copy (returns before it completes) */
+int gf_copy_d2d(gf_ctx* ctx, void* dst, const void* src, int64_t bytes);
 
 #ifdef __cplusplus
 }
