@@ -103,6 +103,11 @@ def main():
                     choices=["kernels", "graph", "persistent"])
     ap.add_argument("--compare", default="all", choices=["all", "none"],
                     help="also time the other FP64 schedules and the mixed solve")
+    ap.add_argument("--forms", default="mixed,batched,grouped,learned",
+                    help="comma list of forms timed against plain FP64 on their own "
+                         "workloads (mixed, batched, grouped, learned) or 'none'")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "gpufem_amd", "weights",
+                                                      "brick8_cantilever_64x32x32.npz"))
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: --gpus %d is not supported (the path would "
@@ -251,6 +256,10 @@ def main():
     }
     lib.gf_dev_free(ctx.handle, d_b)
     lib.gf_dev_free(ctx.handle, d_x)
+    del K, mesh
+    if args.forms != "none":
+        from gpufem_amd import forms
+        out["forms"] = forms.run(ctx, [f for f in args.forms.split(",") if f], args.weights)
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(p, int(round(it_mean)), args.cpu_budget)

@@ -238,7 +238,7 @@ class Context:
 
     def tune(self, ell, force_variant=-1):
         """Pick (or force) the SpMV variant; returns (chosen, ms per variant)."""
-        ms = np.zeros(5)
+        ms = np.zeros(7)
         chosen = C.c_int32()
         self._check(self.lib.gf_ell_tune(self.handle, ell.handle, force_variant, _ptr(ms),
                                          C.byref(chosen)))
@@ -305,6 +305,11 @@ class Group:
         t = _lib.Timing()
         self.ctx._check(self.lib.gf_group_assemble(self.ctx.handle, self.handle, C.byref(t)))
         return t
+
+    def set_E(self, elem_E_list):
+        """New per-element moduli, one array per mesh (structure kept)."""
+        eE = np.ascontiguousarray(np.concatenate(elem_E_list), dtype=np.float64)
+        self.ctx._check(self.lib.gf_group_set_E(self.ctx.handle, self.handle, _ptr(eE)))
 
     def apply_penalty(self, coef=1e4):
         dofs = np.ascontiguousarray(np.concatenate([p.fixed_dofs for p in self.problems]),

@@ -95,6 +95,7 @@ SIGNATURES = {
     "gf_group_create": (C.c_int, [_P, _I32, C.POINTER(MeshDesc), C.POINTER(_P)]),
     "gf_group_info": (C.c_int, [_P, _P, _P, _P]),
     "gf_group_assemble": (C.c_int, [_P, _P, C.POINTER(Timing)]),
+    "gf_group_set_E": (C.c_int, [_P, _P, _P]),
     "gf_group_apply_penalty": (C.c_int, [_P, _P, _P, _P, _D]),
     "gf_group_pcg": (C.c_int, [_P, _P, _P, _P, C.POINTER(SolveOpts), C.POINTER(SolveStats), _P]),
     "gf_group_free": (C.c_int, [_P]),

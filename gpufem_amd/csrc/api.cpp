@@ -218,6 +218,12 @@ int gf_mesh_free(gf_mesh* m) {
 static void ell_release(gf_ell* m) {
   dfree(m->d_val);
   dfree(m->d_col);
+  dfree(m->d_coln);
+  dfree(m->d_bperm);
+  dfree(m->d_bsptr);
+  dfree(m->d_bcol);
+  dfree(m->d_bval);
+  dfree(m->d_bval32);
   dfree(m->d_rownnz);
   dfree(m->d_diag);
   dfree(m->d_color_elems);
@@ -430,6 +436,7 @@ int gf_assemble(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, gf_timing* t) {
   GF_HIP_CHECK(ctx, hipMemcpy(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost));
   m->dinv_valid = false;
   m->val32_valid = false;
+  ++m->val_version;
   if (t) {
     t->ms_device = ms;
     t->ms_host = now_ms() - h0;
@@ -482,6 +489,7 @@ int gf_apply_penalty(gf_ctx* ctx, gf_ell* m, const int32_t* dofs, int64_t nd, do
   }
   m->dinv_valid = false;
   m->val32_valid = false;
+  ++m->val_version;
   return GF_OK;
 }
 
@@ -492,6 +500,11 @@ int gf_spmv(gf_ctx* ctx, const gf_ell* m, const double* x, double* y, int32_t nr
   int st = select_device(ctx);
   if (st) return st;
   const double h0 = now_ms();
+  const int variant = m->plan && m->plan->variant >= 0 ? m->plan->variant : kThreadU8;
+  // the node-column / block-sliced copies are caches of the matrix: refreshing
+  // them does not change the matrix the caller sees
+  if ((st = prepare_variant(ctx, const_cast<gf_ell*>(m), variant))) return st;
+  const SpmvOperands o = spmv_operands(m, variant);
   const size_t vb = size_t(m->n) * size_t(nrhs) * sizeof(double);
   double *dx = nullptr, *dy = nullptr;
   GF_HIP_CHECK(ctx, hipMalloc(&dx, vb));
@@ -500,8 +513,7 @@ int gf_spmv(gf_ctx* ctx, const gf_ell* m, const double* x, double* y, int32_t nr
   if (e == hipSuccess) e = hipMemcpyAsync(dx, x, vb, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipEventRecord(ctx->ev_start, ctx->stream);
   if (e == hipSuccess)
-    e = launch_spmv(ctx->stream, m->plan && m->plan->variant >= 0 ? m->plan->variant : kThreadU8,
-                    m->n, m->d_val, m->d_col, m->d_rownnz, dx, dy, nrhs);
+    e = launch_spmv(ctx->stream, variant, m->n, o.val, o.col, m->d_rownnz, dx, dy, nrhs, &o.bs);
   if (e == hipSuccess) e = hipEventRecord(ctx->ev_stop, ctx->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(y, dy, vb, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);

@@ -86,6 +86,103 @@ __device__ inline T row_dot(int64_t i, int64_t n, const T* __restrict__ val,
   return acc;
 }
 
+// One row over node blocks: slots d j .. d j + d - 1 hold the d DOFs of node
+// coln[i + j n]; summed in slot order, so bitwise equal to row_dot.
+template <int D, class T>
+__device__ inline T row_dot_node(int64_t i, int64_t n, const T* __restrict__ val,
+                                 const int32_t* __restrict__ coln, int32_t rn,
+                                 const T* __restrict__ x) {
+  T acc = T(0);
+  const int32_t nb = rn / D;
+  int32_t j = 0;
+  for (; j + 2 <= nb; j += 2) {
+    T v[2][D];
+    int64_t c[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      c[u] = int64_t(__builtin_nontemporal_load(coln + i + int64_t(j + u) * n)) * D;
+#pragma unroll
+      for (int d = 0; d < D; ++d)
+        v[u][d] = __builtin_nontemporal_load(val + i + int64_t((j + u) * D + d) * n);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int d = 0; d < D; ++d) acc += v[u][d] * x[c[u] + d];
+  }
+  for (; j < nb; ++j) {
+    const int64_t c = int64_t(__builtin_nontemporal_load(coln + i + int64_t(j) * n)) * D;
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+      acc += __builtin_nontemporal_load(val + i + int64_t(j * D + d) * n) * x[c + d];
+  }
+  return acc;
+}
+
+template <class T, int D>
+struct VecOf;
+template <>
+struct VecOf<double, 2> {
+  using type = double2;
+};
+template <>
+struct VecOf<float, 2> {
+  using type = float2;
+};
+
+// the d entries x[c .. c + d - 1] of one node (one vector load when d = 2)
+template <int D, class T>
+__device__ inline void load_node(const T* __restrict__ x, int64_t c, T* xv) {
+  if constexpr (D == 2) {
+    const auto v = *reinterpret_cast<const typename VecOf<T, 2>::type*>(x + c);
+    xv[0] = v.x;
+    xv[1] = v.y;
+  } else {
+#pragma unroll
+    for (int e = 0; e < D; ++e) xv[e] = x[c + e];
+  }
+}
+
+// Node row R of the block-sliced copy, lane l of slice s: acc[r] = (K x)_{dR+r}.
+// Entries are added in ascending column order, as row_dot adds its slots.
+template <int D, class T>
+__device__ inline void bsell_row(const BsellView& bs, const T* __restrict__ bval,
+                                 const int32_t* __restrict__ bcol, int64_t b0, int32_t cnt,
+                                 int lane, const T* __restrict__ x, T* acc) {
+#pragma unroll
+  for (int r = 0; r < D; ++r) acc[r] = T(0);
+  int32_t j = 0;
+  for (; j + 2 <= cnt; j += 2) {
+    T xv[2][D], v[2][D * D];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int64_t b = b0 + j + u;
+      const int64_t c = int64_t(__builtin_nontemporal_load(bcol + b * 64 + lane)) * D;
+#pragma unroll
+      for (int k = 0; k < D * D; ++k)
+        v[u][k] = __builtin_nontemporal_load(bval + (b * (D * D) + k) * 64 + lane);
+      load_node<D, T>(x, c, xv[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int e = 0; e < D; ++e) acc[r] += v[u][r * D + e] * xv[u][e];
+  }
+  if (j < cnt) {
+    const int64_t b = b0 + j;
+    const int64_t c = int64_t(__builtin_nontemporal_load(bcol + b * 64 + lane)) * D;
+    T xv[D];
+    load_node<D, T>(x, c, xv);
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+      for (int e = 0; e < D; ++e)
+        acc[r] += __builtin_nontemporal_load(bval + (b * (D * D) + r * D + e) * 64 + lane) * xv[e];
+  }
+}
+
 // Calls f(i, (K x)_i) for every row of this workgroup's range, with the
 // SpMV variant V (thread per row with 1/4/8 slots in flight, or 2/4 waves
 // sharing a tile of rows).  Every thread of the workgroup must call it.
@@ -93,8 +190,26 @@ template <int V, class T, class F>
 __device__ inline void for_rows_spmv(int64_t n, const T* __restrict__ val,
                                      const int32_t* __restrict__ col,
                                      const int32_t* __restrict__ rownnz,
-                                     const T* __restrict__ x, F&& f) {
-  if constexpr (V <= kThreadU8) {
+                                     const T* __restrict__ x, F&& f, const BsellView& bs) {
+  if constexpr (V == kBsell2 || V == kBsell3) {
+    // four slices (one per wave) per workgroup and sweep, grid-stride
+    constexpr int D = V == kBsell2 ? 2 : 3;
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = int64_t(gridDim.x) * (kNT / 64);
+    for (int64_t sl = int64_t(logical_block()) * (kNT / 64) + (threadIdx.x >> 6); sl < bs.nslices;
+         sl += stride) {
+      const int32_t R = bs.perm[sl * 64 + lane];
+      if (R < 0) continue;
+      const int32_t cnt = rownnz[int64_t(D) * R] / D;
+      T acc[D];
+      bsell_row<D, T>(bs, val, col, bs.sptr[sl], cnt, lane, x, acc);
+#pragma unroll
+      for (int r = 0; r < D; ++r) f(int64_t(D) * R + r, acc[r]);
+    }
+  } else if constexpr (V == kNode2 || V == kNode3) {
+    constexpr int D = V == kNode2 ? 2 : 3;
+    for_my_rows(n, [&](int64_t i) { f(i, row_dot_node<D, T>(i, n, val, col, rownnz[i], x)); });
+  } else if constexpr (V <= kThreadU8) {
     constexpr int U = V == kThreadU1 ? 1 : (V == kThreadU4 ? 4 : 8);
     for_my_rows(n, [&](int64_t i) { f(i, row_dot<U, T>(i, n, val, col, rownnz[i], x)); });
   } else {
@@ -160,6 +275,18 @@ __device__ inline void for_rows_spmv(int64_t n, const T* __restrict__ val,
       break;                                                                              \
     case ::gf::kTile4:                                                                    \
       hipLaunchKernelGGL(KERNEL<::gf::kTile4>, GRID, dim3(::gf::dev::kNT), 0, STREAM, __VA_ARGS__); \
+      break;                                                                              \
+    case ::gf::kNode2:                                                                    \
+      hipLaunchKernelGGL(KERNEL<::gf::kNode2>, GRID, dim3(::gf::dev::kNT), 0, STREAM, __VA_ARGS__); \
+      break;                                                                              \
+    case ::gf::kNode3:                                                                    \
+      hipLaunchKernelGGL(KERNEL<::gf::kNode3>, GRID, dim3(::gf::dev::kNT), 0, STREAM, __VA_ARGS__); \
+      break;                                                                              \
+    case ::gf::kBsell2:                                                                   \
+      hipLaunchKernelGGL(KERNEL<::gf::kBsell2>, GRID, dim3(::gf::dev::kNT), 0, STREAM, __VA_ARGS__); \
+      break;                                                                              \
+    case ::gf::kBsell3:                                                                   \
+      hipLaunchKernelGGL(KERNEL<::gf::kBsell3>, GRID, dim3(::gf::dev::kNT), 0, STREAM, __VA_ARGS__); \
       break;                                                                              \
     default:                                                                              \
       return hipErrorInvalidValue;                                                        \

@@ -75,6 +75,16 @@ struct gf_ell {
   int32_t* d_col = nullptr;
   int32_t* d_rownnz = nullptr;
   int32_t* d_diag = nullptr;  // slot of the diagonal entry per row (-1: none)
+  int32_t* d_coln = nullptr;  // node columns for the node-block SpMV (lazily built)
+  // block-sliced copy for the kBsell SpMV variants (solve.cpp; lazily built)
+  int32_t* d_bperm = nullptr;
+  int64_t* d_bsptr = nullptr;
+  int64_t bslices = 0, bslots = 0;
+  int32_t* d_bcol = nullptr;
+  double* d_bval = nullptr;
+  float* d_bval32 = nullptr;
+  // values change -> val_version++; the copies remember the version they hold
+  uint64_t val_version = 1, bval_version = 0, bval32_version = 0;
   // assembly plan
   int32_t ncolors = 0;
   std::vector<int64_t> color_ptr;     // [ncolors+1] into d_color_elems

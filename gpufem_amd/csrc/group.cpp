@@ -219,6 +219,11 @@ int gf_group_info(const gf_group* g, int64_t* total_dof, int64_t* nnz, int32_t* 
   return GF_OK;
 }
 
+int gf_group_set_E(gf_ctx* ctx, gf_group* g, const double* elem_E) {
+  if (!ctx || !g || !elem_E) return set_error(ctx, GF_ERR_ARG, "gf_group_set_E: null argument");
+  return gf_mesh_set_E(ctx, g->mesh, elem_E);
+}
+
 int gf_group_assemble(gf_ctx* ctx, gf_group* g, gf_timing* t) {
   if (!ctx || !g) return set_error(ctx, GF_ERR_ARG, "gf_group_assemble: null argument");
   return gf_assemble(ctx, g->mesh, g->ell, t);
@@ -272,6 +277,7 @@ int gf_group_apply_penalty(gf_ctx* ctx, gf_group* g, const int32_t* dofs, const 
     return set_error(ctx, GF_ERR_HIP, std::string("gf_group_apply_penalty: ") + hipGetErrorString(e));
   g->ell->dinv_valid = false;
   g->ell->val32_valid = false;
+  ++g->ell->val_version;
   return GF_OK;
 }
 

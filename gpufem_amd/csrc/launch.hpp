@@ -46,15 +46,49 @@ hipError_t launch_penalty_apply(hipStream_t s, double* val, const int32_t* diag,
 
 // ---- SpMV / solver -----------------------------------------------------------------
 // SpMV variants (see solver.hip)
-enum SpmvVariant { kThreadU1 = 0, kThreadU4 = 1, kThreadU8 = 2, kTile2 = 3, kTile4 = 4 };
-constexpr int kNumSpmvVariants = 5;
+// kNode2 / kNode3: thread per row over node blocks of a 2D / 3D FEM matrix,
+// reading one node column per d slots (the "col" argument is then the node
+// column array of gf_ell::d_coln) and the d entries of x of that node together.
+// kBsell2 / kBsell3: thread per node row of the block-sliced copy (BsellView;
+// "val"/"col" are then gf_ell::d_bval / d_bcol): d output rows per thread,
+// one node column and one gather of the node's d x entries per d x d block.
+enum SpmvVariant {
+  kThreadU1 = 0, kThreadU4 = 1, kThreadU8 = 2, kTile2 = 3, kTile4 = 4, kNode2 = 5, kNode3 = 6,
+  kBsell2 = 7, kBsell3 = 8
+};
+constexpr int kNumSpmvVariants = 9;
+inline bool is_node_variant(int v) { return v == kNode2 || v == kNode3; }
+inline bool is_bsell_variant(int v) { return v == kBsell2 || v == kBsell3; }
+// node columns: coln[i + j n] = col[i + d j n] / d for j < width / d
+hipError_t launch_build_coln(hipStream_t s, int64_t n, int32_t width, int d, const int32_t* col,
+                             int32_t* coln);
+
+// Block-sliced ELL ("BSELL"), a solver-side copy of a d-dimensional FEM
+// matrix.  Node rows are sorted by length (descending, stable) inside windows
+// of kBsellWindow node rows and dealt into slices of 64 positions (one
+// wavefront); slice s has sptr[s+1] - sptr[s] block slots, its longest row.
+// Block slot b, lane l: node column bcol[b*64 + l]; block values
+// bval[((b*d + r)*d + e)*64 + l] = K(d R + r, d c + e) for the lane's node
+// row R = perm[s*64 + l] (-1: empty position).  Slots are in ascending column
+// order, so every output row sums its entries in the order of the ELL slots.
+struct BsellView {
+  const int32_t* perm;
+  const int64_t* sptr;
+  int64_t nslices;
+};
+constexpr int kBsellWindow = 2048;
+hipError_t launch_bsell_cols(hipStream_t s, const BsellView& bs, int d, int64_t n,
+                             const int32_t* col, const int32_t* rownnz, int32_t* bcol);
+hipError_t launch_bsell_vals(hipStream_t s, const BsellView& bs, int d, int64_t n,
+                             const double* val, const int32_t* rownnz, double* bval);
 constexpr int kMaxPcgBlocks = 2048;
 
 // workgroups of the standalone SpMV and the PCG kernels for n rows
 int spmv_blocks(int64_t n);
+// bs: the block-sliced view (used by the kBsell variants only; may be null)
 hipError_t launch_spmv(hipStream_t s, int variant, int64_t n, const double* val,
                        const int32_t* col, const int32_t* rownnz, const double* x, double* y,
-                       int nrhs);
+                       int nrhs, const BsellView* bs = nullptr);
 hipError_t launch_jacobi_inv(hipStream_t s, int64_t n, const double* val,
                              const int32_t* diag, double* dinv);
 
@@ -78,6 +112,7 @@ struct PcgBuffers {
   double* part2;  // per-block partials of (r.z, r.r[, b.b])
   int nblocks;    // workgroups of every PCG kernel (<= kMaxPcgBlocks)
   PcgState* st;
+  BsellView bs;   // block-sliced view (kBsell variants)
 };
 size_t pcg_state_bytes();
 size_t grid_barrier_bytes();

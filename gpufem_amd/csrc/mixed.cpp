@@ -12,6 +12,7 @@ struct gf_mixed_ws {
   double* q64 = nullptr;
   double* part3 = nullptr;
   gf::MixedState* st = nullptr;
+  uint64_t dinv_version = 0;  // matrix values the FP32 Jacobi copy was made from
 };
 
 namespace gf {
@@ -58,24 +59,44 @@ int run_mixed(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, PcgHostState* h
     }
   }
   gf_mixed_ws* W = P->mixed;
-  const size_t slots = size_t(n) * size_t(m->width);
-  if (!m->d_val32) GF_HIP_CHECK(ctx, hipMalloc(&m->d_val32, slots * sizeof(float)));
-  if (!m->val32_valid) {
-    GF_HIP_CHECK(ctx, launch_to_f32(ctx->stream, int64_t(slots), m->d_val, m->d_val32));
+  int st = prepare_variant(ctx, m, P->variant);
+  if (st) return st;
+  const SpmvOperands o = spmv_operands(m, P->variant);
+  const float* val32 = nullptr;
+  if (is_bsell_variant(P->variant)) {
+    // FP32 copy of the block-sliced values
+    const size_t cnt = size_t(m->bslots) * 64 * m->dim * m->dim;
+    if (!m->d_bval32) GF_HIP_CHECK(ctx, hipMalloc(&m->d_bval32, cnt * sizeof(float)));
+    if (m->bval32_version != m->val_version) {
+      GF_HIP_CHECK(ctx, launch_to_f32(ctx->stream, int64_t(cnt), m->d_bval, m->d_bval32));
+      m->bval32_version = m->val_version;
+    }
+    val32 = m->d_bval32;
+  } else {
+    const size_t slots = size_t(n) * size_t(m->width);
+    if (!m->d_val32) GF_HIP_CHECK(ctx, hipMalloc(&m->d_val32, slots * sizeof(float)));
+    if (!m->val32_valid) {
+      GF_HIP_CHECK(ctx, launch_to_f32(ctx->stream, int64_t(slots), m->d_val, m->d_val32));
+      m->val32_valid = true;
+    }
+    val32 = m->d_val32;
+  }
+  if (W->dinv_version != m->val_version) {
     GF_HIP_CHECK(ctx, launch_to_f32(ctx->stream, n, m->d_dinv, W->dinv32));
-    m->val32_valid = true;
+    W->dinv_version = m->val_version;
   }
   MixedBuffers B;
   B.n = n;
   B.variant = P->variant;
   B.nblocks = P->nblocks;
-  B.val = m->d_val;
-  B.col = m->d_col;
+  B.val = o.val;
+  B.col = o.col;
+  B.bs = o.bs;
   B.rownnz = m->d_rownnz;
   B.b = P->wb;
   B.x = P->wx;
   B.q64 = W->q64;
-  B.val32 = m->d_val32;
+  B.val32 = val32;
   B.dinv32 = W->dinv32;
   B.r32 = W->r32;
   B.z32 = W->z32;

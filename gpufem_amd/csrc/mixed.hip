@@ -105,7 +105,7 @@ __global__ __launch_bounds__(kNT) void k_m_spmv_pq(MixedBuffers B) {
   for_rows_spmv<V, float>(B.n, B.val32, B.col, B.rownnz, B.p32, [&](int64_t i, float qi) {
     B.q32[i] = qi;
     pq += double(B.p32[i]) * double(qi);
-  });
+  }, B.bs);
   pq = block_sum(pq, sh);
   if (threadIdx.x == 0) B.part1[blockIdx.x] = pq;
 }
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(kNT) void k_m_refresh(MixedBuffers B, int parity) {
     B.z32[i] = z32;
     rz2 += double(r32) * double(z32);
     rr2 += ri * ri;
-  });
+  }, B.bs);
   rz2 = block_sum(rz2, sh);
   rr2 = block_sum(rr2, sh);
   if (threadIdx.x == 0) {
@@ -230,7 +230,7 @@ hipError_t launch_mixed_setup(hipStream_t s, const MixedBuffers& B, int use_x0, 
                               double delta, int maxit) {
   hipError_t e;
   if (use_x0) {
-    e = launch_spmv(s, B.variant, B.n, B.val, B.col, B.rownnz, B.x, B.q64, 1);
+    e = launch_spmv(s, B.variant, B.n, B.val, B.col, B.rownnz, B.x, B.q64, 1, &B.bs);
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_m_init, dim3(B.nblocks), dim3(kNT), 0, s, B, use_x0);

@@ -32,6 +32,7 @@ struct MixedBuffers {
   double* part2;  // r.z, r.r of the update
   double* part3;  // r.z, r.r of a refreshed residual
   MixedState* st;
+  BsellView bs;   // block-sliced view (kBsell variants: val, val32, col are its arrays)
 };
 
 size_t mixed_state_bytes();

@@ -101,14 +101,120 @@ void set_variant(gf_ell* m, int variant) {
   P->nblocks = std::max(1, nb);
 }
 
+// The node-block / block-sliced variants of the matrix's dimension, or -1
+// (matrices made by gf_ell_upload carry no node structure).
+int node_variant(const gf_ell* m) {
+  if (m->dim == 2 && m->width % 2 == 0) return kNode2;
+  if (m->dim == 3 && m->width % 3 == 0) return kNode3;
+  return -1;
+}
+
+int bsell_variant(const gf_ell* m) {
+  if (m->dim == 2 && m->width % 2 == 0 && m->n % 2 == 0) return kBsell2;
+  if (m->dim == 3 && m->width % 3 == 0 && m->n % 3 == 0) return kBsell3;
+  return -1;
+}
+
+bool variant_fits(const gf_ell* m, int v) {
+  if (v < 0 || v >= kNumSpmvVariants) return false;
+  if (is_node_variant(v)) return v == node_variant(m);
+  if (is_bsell_variant(v)) return v == bsell_variant(m);
+  return true;
+}
+
+int ensure_coln(gf_ctx* ctx, gf_ell* m) {
+  if (m->d_coln) return GF_OK;
+  const int d = m->dim;
+  GF_HIP_CHECK(ctx, hipMalloc(&m->d_coln, size_t(m->n) * (m->width / d) * sizeof(int32_t)));
+  GF_HIP_CHECK(ctx, launch_build_coln(ctx->stream, m->n, m->width, d, m->d_col, m->d_coln));
+  return GF_OK;
+}
+
+BsellView bsell_view(const gf_ell* m) { return BsellView{m->d_bperm, m->d_bsptr, m->bslices}; }
+
+// Structure of the block-sliced copy: node rows sorted by length inside
+// windows, slices of 64 positions, one block slot per entry of the longest
+// row of the slice; then the node columns.
+int ensure_bsell(gf_ctx* ctx, gf_ell* m) {
+  if (m->d_bperm) return GF_OK;
+  const int d = m->dim;
+  const int64_t N = m->n / d;
+  std::vector<int32_t> rn(static_cast<size_t>(m->n));
+  GF_HIP_CHECK(ctx, hipMemcpy(rn.data(), m->d_rownnz, size_t(m->n) * 4, hipMemcpyDeviceToHost));
+  std::vector<int32_t> cnt(static_cast<size_t>(N)), perm;
+  for (int64_t R = 0; R < N; ++R) cnt[size_t(R)] = rn[size_t(d * R)] / d;
+  const int64_t nwin = (N + kBsellWindow - 1) / kBsellWindow;
+  std::vector<int64_t> sptr(1, 0);
+  std::vector<int32_t> order;
+  for (int64_t w = 0; w < nwin; ++w) {
+    const int64_t r0 = w * kBsellWindow, r1 = std::min<int64_t>(N, r0 + kBsellWindow);
+    order.resize(size_t(r1 - r0));
+    for (int64_t R = r0; R < r1; ++R) order[size_t(R - r0)] = int32_t(R);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return cnt[size_t(a)] > cnt[size_t(b)]; });
+    for (size_t s0 = 0; s0 < order.size(); s0 += 64) {
+      for (size_t l = 0; l < 64; ++l) perm.push_back(s0 + l < order.size() ? order[s0 + l] : -1);
+      sptr.push_back(sptr.back() + cnt[size_t(order[s0])]);
+    }
+  }
+  m->bslices = int64_t(sptr.size()) - 1;
+  m->bslots = sptr.back();
+  GF_HIP_CHECK(ctx, hipMalloc(&m->d_bperm, perm.size() * 4));
+  GF_HIP_CHECK(ctx, hipMalloc(&m->d_bsptr, sptr.size() * 8));
+  GF_HIP_CHECK(ctx, hipMalloc(&m->d_bcol, size_t(m->bslots) * 64 * 4));
+  GF_HIP_CHECK(ctx, hipMalloc(&m->d_bval, size_t(m->bslots) * 64 * d * d * sizeof(double)));
+  GF_HIP_CHECK(ctx, hipMemcpy(m->d_bperm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice));
+  GF_HIP_CHECK(ctx, hipMemcpy(m->d_bsptr, sptr.data(), sptr.size() * 8, hipMemcpyHostToDevice));
+  GF_HIP_CHECK(ctx, launch_bsell_cols(ctx->stream, bsell_view(m), d, m->n, m->d_col, m->d_rownnz,
+                                      m->d_bcol));
+  m->bval_version = 0;
+  m->bval32_version = 0;
+  return GF_OK;
+}
+
+// Values of the copy follow the ELL values (assembly and penalty bump the version).
+int ensure_bsell_vals(gf_ctx* ctx, gf_ell* m) {
+  int st = ensure_bsell(ctx, m);
+  if (st) return st;
+  if (m->bval_version != m->val_version) {
+    GF_HIP_CHECK(ctx, launch_bsell_vals(ctx->stream, bsell_view(m), m->dim, m->n, m->d_val,
+                                        m->d_rownnz, m->d_bval));
+    m->bval_version = m->val_version;
+  }
+  return GF_OK;
+}
+
+int prepare_variant(gf_ctx* ctx, gf_ell* m, int v) {
+  if (is_node_variant(v)) return ensure_coln(ctx, m);
+  if (is_bsell_variant(v)) return ensure_bsell_vals(ctx, m);
+  return GF_OK;
+}
+
+SpmvOperands spmv_operands(const gf_ell* m, int variant) {
+  SpmvOperands o;
+  o.val = m->d_val;
+  o.col = m->d_col;
+  o.bs = BsellView{nullptr, nullptr, 0};
+  if (is_node_variant(variant)) {
+    o.col = m->d_coln;
+  } else if (is_bsell_variant(variant)) {
+    o.val = m->d_bval;
+    o.col = m->d_bcol;
+    o.bs = bsell_view(m);
+  }
+  return o;
+}
+
 PcgBuffers make_buffers(gf_ell* m, const double* dinv) {
   gf_solve_plan* P = m->plan;
+  const SpmvOperands o = spmv_operands(m, P->variant);
   PcgBuffers B;
   B.n = m->n;
   B.width = m->width;
   B.variant = P->variant;
-  B.val = m->d_val;
-  B.col = m->d_col;
+  B.val = o.val;
+  B.col = o.col;
+  B.bs = o.bs;
   B.rownnz = m->d_rownnz;
   B.dinv = dinv;
   B.b = P->wb;
@@ -130,6 +236,9 @@ int tune_spmv(gf_ctx* ctx, gf_ell* m, int force, double* ms_out, int* chosen) {
   gf_solve_plan* P = m->plan;
   if (force >= 0) {
     if (force >= kNumSpmvVariants) return set_error(ctx, GF_ERR_ARG, "gf_ell_tune: bad variant");
+    if (!variant_fits(m, force))
+      return set_error(ctx, GF_ERR_UNSUPPORTED, "gf_ell_tune: the variant does not fit the matrix");
+    if ((st = prepare_variant(ctx, m, force))) return st;
     set_variant(m, force);
     if (chosen) *chosen = force;
     return GF_OK;
@@ -139,6 +248,11 @@ int tune_spmv(gf_ctx* ctx, gf_ell* m, int force, double* ms_out, int* chosen) {
   double best = 1e300;
   int best_v = kThreadU8;
   for (int v = 0; v < kNumSpmvVariants; ++v) {
+    if (!variant_fits(m, v)) {
+      if (ms_out) ms_out[v] = -1.0;
+      continue;
+    }
+    if ((st = prepare_variant(ctx, m, v))) return st;
     set_variant(m, v);
     PcgBuffers B = make_buffers(m, nullptr);
     GF_HIP_CHECK(ctx, launch_spmv_pq_repeat(ctx->stream, B, 2));
@@ -287,6 +401,7 @@ int gf_pcg(gf_ctx* ctx, gf_ell* m, const double* b, double* x, int32_t nrhs,
   }
   const double* dinv = opts.precond == GF_PRECOND_JACOBI ? m->d_dinv : nullptr;
   const int key = dinv ? 1 : 0;
+  if ((st = prepare_variant(ctx, m, P->variant))) return st;
   PcgBuffers B = make_buffers(m, dinv);
   int sched = opts.schedule == GF_SCHED_AUTO ? GF_SCHED_GRAPH : opts.schedule;
   if (sched == GF_SCHED_PERSISTENT && !P->persistent_ok)
@@ -420,6 +535,7 @@ int gf_profile_spmv(gf_ctx* ctx, gf_ell* m, int32_t nrep, double* ms_avg) {
   int st = ensure_plan(ctx, m);
   if (st) return st;
   if (m->plan->variant < 0 && (st = tune_spmv(ctx, m, -1, nullptr, nullptr))) return st;
+  if ((st = prepare_variant(ctx, m, m->plan->variant))) return st;
   PcgBuffers B = make_buffers(m, nullptr);
   GF_HIP_CHECK(ctx, hipMemsetAsync(m->plan->p, 0, size_t(m->n) * sizeof(double), ctx->stream));
   GF_HIP_CHECK(ctx, launch_spmv_pq_repeat(ctx->stream, B, 1));

@@ -32,6 +32,16 @@ int ensure_plan(gf_ctx* ctx, gf_ell* m);
 void set_variant(gf_ell* m, int variant);
 int tune_spmv(gf_ctx* ctx, gf_ell* m, int force, double* ms_out, int* chosen);
 PcgBuffers make_buffers(gf_ell* m, const double* dinv);
+// arrays the SpMV variant reads: the ELL arrays, the node columns (kNode) or
+// the block-sliced copy (kBsell)
+struct SpmvOperands {
+  const double* val;
+  const int32_t* col;
+  BsellView bs;
+};
+SpmvOperands spmv_operands(const gf_ell* m, int variant);
+// builds what the variant reads and refreshes copies of stale values
+int prepare_variant(gf_ctx* ctx, gf_ell* m, int variant);
 int prepare_jacobi(gf_ctx* ctx, gf_ell* m);
 int run_pcg(gf_ctx* ctx, gf_ell* m, const PcgBuffers& B, int sched, int key, int use_x0,
             double rtol, int maxit, int verify, PcgHostState* h);

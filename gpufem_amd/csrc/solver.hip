@@ -1,9 +1,9 @@
 // ELL SpMV and the Jacobi-preconditioned CG in FP64.
 //
-// Rows are split into contiguous blocks, one per workgroup, and workgroup
-// ids are remapped so that each XCD (workgroups are dispatched round-robin
-// over the 8 XCDs) owns a contiguous 1/8 of the rows: the x-gathers of
-// neighbouring rows then hit the same L2.
+// Rows are visited in 256-row tiles in grid-stride order, with workgroup
+// ids remapped so that each XCD (workgroups are dispatched round-robin over
+// the 8 XCDs) owns a contiguous part of every window of tiles: the
+// x-gathers of neighbouring rows then hit the same L2.
 //
 // SpMV variants (chosen per matrix by the tuner in solve.cpp):
 //   kThreadU1/U4/U8  one thread per row, slots streamed 1/4/8 at a time
@@ -12,6 +12,10 @@
 //                    identical.
 //   kTile2/kTile4    2 or 4 waves share a 64-row tile, each taking every
 //                    2nd/4th slot; partial sums combined through LDS.
+//   kNode2/kNode3    thread per row over the d x d node blocks of a 2D / 3D
+//                    FEM matrix: one node column per d slots and the d
+//                    entries of x read together (fewer loads, 4/d instead of
+//                    4 column bytes per entry); bitwise equal to kThreadU1.
 //
 // CG schedules: three launches per iteration (k_spmv_pq, k_update,
 // k_direction) with all scalars in a device PcgState, so the host can
@@ -56,8 +60,51 @@ __global__ __launch_bounds__(kNT) void k_spmv(int64_t n, const double* __restric
                                               const int32_t* __restrict__ col,
                                               const int32_t* __restrict__ rownnz,
                                               const double* __restrict__ x,
-                                              double* __restrict__ y) {
-  for_rows_spmv<V, double>(n, val, col, rownnz, x, [&](int64_t i, double v) { y[i] = v; });
+                                              double* __restrict__ y, BsellView bs) {
+  for_rows_spmv<V, double>(n, val, col, rownnz, x, [&](int64_t i, double v) { y[i] = v; }, bs);
+}
+
+// block-sliced copy: node columns (padding blocks point at the row's own
+// node, whose x entries exist) and values (padding blocks are zero)
+__global__ void k_bsell_cols(BsellView bs, int d, int64_t n, const int32_t* __restrict__ col,
+                             const int32_t* __restrict__ rownnz, int32_t* __restrict__ bcol) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < bs.nslices * 64;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t sl = t >> 6;
+    const int lane = int(t & 63);
+    const int32_t R = bs.perm[t];
+    const int64_t b0 = bs.sptr[sl], w = bs.sptr[sl + 1] - b0;
+    const int32_t cnt = R >= 0 ? rownnz[int64_t(d) * R] / d : 0;
+    for (int64_t j = 0; j < w; ++j)
+      bcol[(b0 + j) * 64 + lane] =
+          j < cnt ? col[int64_t(d) * R + j * d * n] / d : (R >= 0 ? R : 0);
+  }
+}
+
+__global__ void k_bsell_vals(BsellView bs, int d, int64_t n, const double* __restrict__ val,
+                             const int32_t* __restrict__ rownnz, double* __restrict__ bval) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < bs.nslices * 64;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t sl = t >> 6;
+    const int lane = int(t & 63);
+    const int32_t R = bs.perm[t];
+    const int64_t b0 = bs.sptr[sl], w = bs.sptr[sl + 1] - b0;
+    const int32_t cnt = R >= 0 ? rownnz[int64_t(d) * R] / d : 0;
+    for (int64_t j = 0; j < w; ++j)
+      for (int r = 0; r < d; ++r)
+        for (int e = 0; e < d; ++e)
+          bval[(((b0 + j) * d + r) * d + e) * 64 + lane] =
+              j < cnt ? val[int64_t(d) * R + r + (j * d + e) * n] : 0.0;
+  }
+}
+
+__global__ void k_build_coln(int64_t n, int32_t nbw, int d, const int32_t* col, int32_t* coln) {
+  const int64_t total = n * nbw;
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t j = t / n, i = t - j * n;
+    coln[t] = col[i + j * d * n] / d;
+  }
 }
 
 __global__ void k_jacobi_inv(int64_t n, const double* val, const int32_t* diag, double* dinv) {
@@ -78,7 +125,7 @@ __device__ inline double phase_spmv_pq(const PcgBuffers& B) {
   for_rows_spmv<V, double>(B.n, B.val, B.col, B.rownnz, B.p, [&](int64_t i, double qi) {
     B.q[i] = qi;
     pq += B.p[i] * qi;
-  });
+  }, B.bs);
   return pq;
 }
 
@@ -359,14 +406,41 @@ int spmv_blocks(int64_t n) {
   return int(nb);
 }
 
+hipError_t launch_build_coln(hipStream_t s, int64_t n, int32_t width, int d, const int32_t* col,
+                             int32_t* coln) {
+  if (d < 2 || width % d) return hipErrorInvalidValue;
+  const int64_t total = n * (width / d);
+  int64_t nb = (total + kNT - 1) / kNT;
+  if (nb > 8192) nb = 8192;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(k_build_coln, dim3(unsigned(nb)), dim3(kNT), 0, s, n, width / d, d, col, coln);
+  return hipGetLastError();
+}
+
+hipError_t launch_bsell_cols(hipStream_t s, const BsellView& bs, int d, int64_t n,
+                             const int32_t* col, const int32_t* rownnz, int32_t* bcol) {
+  hipLaunchKernelGGL(k_bsell_cols, dim3(grid_for(bs.nslices * 64, kNT, 8192)), dim3(kNT), 0, s, bs,
+                     d, n, col, rownnz, bcol);
+  return hipGetLastError();
+}
+
+hipError_t launch_bsell_vals(hipStream_t s, const BsellView& bs, int d, int64_t n,
+                             const double* val, const int32_t* rownnz, double* bval) {
+  hipLaunchKernelGGL(k_bsell_vals, dim3(grid_for(bs.nslices * 64, kNT, 8192)), dim3(kNT), 0, s, bs,
+                     d, n, val, rownnz, bval);
+  return hipGetLastError();
+}
+
 hipError_t launch_spmv(hipStream_t s, int variant, int64_t n, const double* val,
                        const int32_t* col, const int32_t* rownnz, const double* x, double* y,
-                       int nrhs) {
+                       int nrhs, const BsellView* bs) {
   const dim3 g(spmv_blocks(n));
+  const BsellView view = bs ? *bs : BsellView{nullptr, nullptr, 0};
+  if (is_bsell_variant(variant) && !view.perm) return hipErrorInvalidValue;
   for (int k = 0; k < nrhs; ++k) {
     const double* xk = x + int64_t(k) * n;
     double* yk = y + int64_t(k) * n;
-    GF_DISPATCH_VARIANT(variant, k_spmv, g, s, n, val, col, rownnz, xk, yk);
+    GF_DISPATCH_VARIANT(variant, k_spmv, g, s, n, val, col, rownnz, xk, yk, view);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -384,7 +458,7 @@ hipError_t launch_pcg_setup(hipStream_t s, const PcgBuffers& B, int use_x0, doub
                             int maxit) {
   hipError_t e;
   if (use_x0) {
-    e = launch_spmv(s, B.variant, B.n, B.val, B.col, B.rownnz, B.x, B.q, 1);
+    e = launch_spmv(s, B.variant, B.n, B.val, B.col, B.rownnz, B.x, B.q, 1, &B.bs);
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_pcg_init, dim3(B.nblocks), dim3(kNT), 0, s, B, use_x0);
@@ -413,6 +487,10 @@ int persistent_blocks(int variant, int num_cu) {
     case kThreadU8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_persistent<kThreadU8>, kNT, 0); break;
     case kTile2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_persistent<kTile2>, kNT, 0); break;
     case kTile4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_persistent<kTile4>, kNT, 0); break;
+    case kNode2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_persistent<kNode2>, kNT, 0); break;
+    case kNode3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_persistent<kNode3>, kNT, 0); break;
+    case kBsell2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_persistent<kBsell2>, kNT, 0); break;
+    case kBsell3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_persistent<kBsell3>, kNT, 0); break;
     default: break;
   }
   if (e != hipSuccess || per_cu < 1) return 0;
@@ -436,6 +514,10 @@ hipError_t launch_pcg_persistent(hipStream_t s, const PcgBuffers& B, void* barri
     case kThreadU8: return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent<kThreadU8>), g, blk, args, 0, s);
     case kTile2: return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent<kTile2>), g, blk, args, 0, s);
     case kTile4: return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent<kTile4>), g, blk, args, 0, s);
+    case kNode2: return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent<kNode2>), g, blk, args, 0, s);
+    case kNode3: return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent<kNode3>), g, blk, args, 0, s);
+    case kBsell2: return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent<kBsell2>), g, blk, args, 0, s);
+    case kBsell3: return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent<kBsell3>), g, blk, args, 0, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -466,7 +548,7 @@ hipError_t launch_spmv_pq_repeat(hipStream_t s, const PcgBuffers& B, int nrep) {
 }
 
 hipError_t launch_true_residual(hipStream_t s, const PcgBuffers& B, double* out_dev) {
-  hipError_t e = launch_spmv(s, B.variant, B.n, B.val, B.col, B.rownnz, B.x, B.q, 1);
+  hipError_t e = launch_spmv(s, B.variant, B.n, B.val, B.col, B.rownnz, B.x, B.q, 1, &B.bs);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_resid_partial, dim3(B.nblocks), dim3(kNT), 0, s, B);
   if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -1,0 +1,169 @@
+"""The three forms beside plain FP64 PCG, each timed against plain FP64 PCG
+on the same meshes (one GPU, vectors resident in HBM):
+
+  mixed    Brick8 128^3 cube, lognormal E: FP32 inner PCG in FP64 iterative
+           refinement vs FP64 Jacobi-PCG; displacement difference reported
+  batched  Brick20 48^3 cube, 32 random top-face load cases: one block PCG
+           (ELL SpMM per iteration) vs the same load cases one after another
+  grouped  256 Q4 plates of uneven sizes (16x8 .. 256x128): one grouped
+           assembly + segmented PCG vs plate after plate
+  learned  Brick8 64x32x32 cantilevers: PCG from the network's prediction vs
+           from zero, on held-out members
+
+Every solve stops at ||b - K u|| <= 1e-8 ||b|| (true residual, checked).
+"""
+import ctypes as C
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import DevBuf, _lib, solve_opts
+from . import problems as P
+
+RTOL = 1e-8
+
+
+def _log(msg):
+    print("[forms] " + msg, file=sys.stderr, flush=True)
+
+
+def _setup(ctx, p):
+    mesh = ctx.upload_mesh(p.elem_type, p.ngauss, p.conn, p.coords, p.E, p.nu, elem_E=p.elem_E)
+    K = ctx.symbolic(mesh)
+    ta = ctx.assemble(mesh, K)
+    ctx.apply_penalty(K, p.fixed_dofs)
+    ctx.tune(K)
+    return mesh, K, ta
+
+
+def _solve(ctx, K, d_b, d_x, nrhs=1, **kw):
+    o = solve_opts(device_ptrs=1, rtol=RTOL, **kw)
+    st = _lib.SolveStats()
+    ctx._check(ctx.lib.gf_pcg(ctx.handle, K.handle, d_b.ptr, d_x.ptr, nrhs, C.byref(o),
+                              C.byref(st)))
+    return st
+
+
+def _stats(st):
+    return {"iterations": int(st.iterations), "seconds": st.ms_device / 1e3,
+            "true_relres": float(st.true_relres)}
+
+
+def form_mixed(ctx, n=128):
+    p = P.config_brick8_cube(n)
+    _log("mixed: %s, %d DOF" % (p.name, p.ndof))
+    mesh, K, _ = _setup(ctx, p)
+    d_b, d_x = DevBuf.from_array(ctx, p.f), DevBuf(ctx, p.ndof * 8)
+    st64 = _solve(ctx, K, d_b, d_x)
+    u64 = d_x.download(np.float64)
+    _log("mixed: FP64 %d it %.3f s" % (st64.iterations, st64.ms_device / 1e3))
+    st32 = _solve(ctx, K, d_b, d_x, precision=_lib.GF_PREC_MIXED)
+    u32 = d_x.download(np.float64)
+    _log("mixed: mixed %d it (%d refinements, %d fallbacks) %.3f s" % (
+        st32.iterations, st32.refinements, st32.fallbacks, st32.ms_device / 1e3))
+    r = {"workload": p.name, "dof": p.ndof, "nnz": K.nnz, "fp64": _stats(st64),
+         "mixed": dict(_stats(st32), refinements=int(st32.refinements),
+                       fallbacks=int(st32.fallbacks)),
+         "displacement_rel_diff_vs_fp64": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))}
+    r["speedup"] = r["fp64"]["seconds"] / r["mixed"]["seconds"]
+    return r
+
+
+def form_batched(ctx, n=48, nrhs=32, sequential_sample=4):
+    p = P.config_brick20_cube(n)
+    F = P.random_top_loads(p, nrhs, P.SEED_BRICK20_RHS)
+    _log("batched: %s, %d DOF x %d load cases" % (p.name, p.ndof, nrhs))
+    mesh, K, _ = _setup(ctx, p)
+    d_b = DevBuf.from_array(ctx, np.ascontiguousarray(F))
+    d_x = DevBuf(ctx, F.nbytes)
+    stb = _solve(ctx, K, d_b, d_x, nrhs=nrhs, batched=1)
+    Xb = d_x.download(np.float64).reshape(nrhs, -1)
+    _log("batched: block PCG %d it %.3f s" % (stb.iterations, stb.ms_device / 1e3))
+    seq_s, seq_it, worst = 0.0, [], 0.0
+    d_b1, d_x1 = DevBuf(ctx, p.ndof * 8), DevBuf(ctx, p.ndof * 8)
+    for k in range(sequential_sample):
+        d_b1.upload(F[k])
+        st = _solve(ctx, K, d_b1, d_x1)
+        seq_s += st.ms_device / 1e3
+        seq_it.append(int(st.iterations))
+        xk = d_x1.download(np.float64)
+        worst = max(worst, float(np.linalg.norm(Xb[k] - xk) / np.linalg.norm(xk)))
+        _log("batched: single load case %d: %d it %.3f s" % (k, st.iterations, st.ms_device / 1e3))
+    per_rhs = seq_s / sequential_sample
+    r = {"workload": p.name + "_%drhs" % nrhs, "dof": p.ndof, "nnz": K.nnz, "nrhs": nrhs,
+         "block": _stats(stb),
+         "sequential": {"seconds_per_rhs": per_rhs, "iterations": seq_it,
+                        "seconds_all_rhs_extrapolated": per_rhs * nrhs,
+                        "sample": "first %d of %d load cases" % (sequential_sample, nrhs)},
+         "block_vs_single_max_rel_diff": worst}
+    r["speedup"] = per_rhs * nrhs / r["block"]["seconds"]
+    return r
+
+
+def form_grouped(ctx, count=256):
+    sizes = P.q4_batch_sizes(count)
+    probs = [P.structured(P.GF_Q4, nx, ny, ngauss=2, name="q4_%dx%d" % (nx, ny))
+             for nx, ny in sizes]
+    ndof = sum(q.ndof for q in probs)
+    _log("grouped: %d Q4 plates, %d DOF in all" % (count, ndof))
+    G = ctx.group(probs)
+    ta = G.assemble()
+    G.apply_penalty()
+    x, st, its = G.pcg(rtol=RTOL)
+    _log("grouped: one launch sequence: assembly %.3f ms, PCG %d it max %.3f s" % (
+        ta.ms_device, st.iterations, st.ms_device / 1e3))
+    seq_asm, seq_pcg, worst = 0.0, 0.0, 0.0
+    xs = G.split(x)
+    for k, q in enumerate(probs):
+        mesh = ctx.upload_mesh(q.elem_type, q.ngauss, q.conn, q.coords, q.E, q.nu)
+        K = ctx.symbolic(mesh)
+        t = ctx.assemble(mesh, K)
+        ctx.apply_penalty(K, q.fixed_dofs)
+        d_b, d_x = DevBuf.from_array(ctx, q.f), DevBuf(ctx, q.ndof * 8)
+        s1 = _solve(ctx, K, d_b, d_x)
+        seq_asm += t.ms_device / 1e3
+        seq_pcg += s1.ms_device / 1e3
+        xk = d_x.download(np.float64)
+        worst = max(worst, float(np.linalg.norm(xs[k] - xk) / np.linalg.norm(xk)))
+    _log("grouped: plate after plate: assembly %.3f s, PCG %.3f s" % (seq_asm, seq_pcg))
+    r = {"workload": "q4_plates_%d_16x8_to_256x128" % count, "dof": ndof, "meshes": count,
+         "grouped": {"assembly_s": ta.ms_device / 1e3, "pcg_s": st.ms_device / 1e3,
+                     "iterations_max": int(its.max()), "true_relres": float(st.true_relres)},
+         "sequential": {"assembly_s": seq_asm, "pcg_s": seq_pcg},
+         "grouped_vs_single_max_rel_diff": worst}
+    r["speedup"] = (seq_asm + seq_pcg) / (r["grouped"]["assembly_s"] + r["grouped"]["pcg_s"])
+    return r
+
+
+def form_learned(ctx, weights, members=16):
+    from . import learned as L
+    if not os.path.exists(weights):
+        return {"skipped": "no trained weights at %s" % weights}
+    net, c_u, meta = L.load_weights(ctx, weights, batch_max=1)
+    fam = L.Family()
+    ks = list(range(L.HOLDOUT_BASE, L.HOLDOUT_BASE + members))
+    _log("learned: %d held-out cantilevers" % members)
+    recs = L.evaluate(ctx, fam, net, c_u, ks, rtol=RTOL, log=_log)
+    s = L.summarize(recs)
+    s["workload"] = "brick8_cantilever_64x32x32"
+    s["trained_on"] = meta.get("train_members")
+    s["speedup"] = 1.0 / s["time_ratio"]
+    return s
+
+
+FORMS = {"mixed": form_mixed, "batched": form_batched, "grouped": form_grouped}
+
+
+def run(ctx, names, weights):
+    out = {}
+    for name in names:
+        t0 = time.perf_counter()
+        try:
+            out[name] = form_learned(ctx, weights) if name == "learned" else FORMS[name](ctx)
+        except Exception as e:  # report, keep the other forms
+            out[name] = {"error": "%s: %s" % (type(e).__name__, e)}
+        out[name]["wall_s"] = time.perf_counter() - t0
+        _log("%s done in %.1f s" % (name, out[name]["wall_s"]))
+    return out

@@ -1,0 +1,325 @@
+"""Learned warm start for the Brick8 cantilever family.
+
+The solver generates its own training data: for members k of the family
+(gpufem_amd.problems.cantilever_problem: random modulus field, random top
+traction, x = 0 clamped) the grouped FP64 Jacobi-PCG computes displacement
+fields; the network of libgpufem (gf_nn_*, a BF16 3D U-Net on the node grid
+with hand-written HIP forward, backward and Adam kernels) learns
+features -> displacement; its prediction becomes the PCG initial guess.
+Zero-start PCG is the fallback and the judge: every warm-started solve must
+reach the same true residual ||b - K u|| <= rtol ||b||.
+
+Features per node (16 channels, bf16):
+  0      log(E / 210000) / sigma, averaged over the node's elements
+  1..3   nodal force / s_f  (s_f = RMS of the force on the top-face nodes)
+  4      1 on the clamped face, 0 elsewhere (the prediction is zeroed there)
+  5..7   x / nx, y / ny, z / nz
+  8..15  zero
+Target: u / (s_f c_u), with c_u the RMS of u / s_f over the training set.
+
+Usage (trains, evaluates, writes the weights file):
+  python -m gpufem_amd.learned --train 2000 --holdout 256 --epochs 20 --out W.npz
+"""
+import argparse
+import ctypes as C
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import DevBuf, Net, _ptr, problems
+from ._lib import GF_OK  # noqa: F401
+
+MASK_CHANNEL = 4
+NCH = 16
+HOLDOUT_BASE = 1_000_000  # member index of the first held-out sample
+
+
+def bf16_bits(a):
+    """float32 -> bfloat16 bit patterns (round to nearest even)."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+class Family:
+    """Geometry shared by the members of one cantilever family."""
+
+    def __init__(self, nx=64, ny=32, nz=32):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.grid = (nx + 1, ny + 1, nz + 1)  # node grid (x, y, z)
+        self.nodes = (nx + 1) * (ny + 1) * (nz + 1)
+        self.template = problems.cantilever_problem(0, nx, ny, nz)
+        z, y, x = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1),
+                              indexing="ij")
+        base = np.zeros((nz + 1, ny + 1, nx + 1, NCH), dtype=np.float32)
+        base[..., MASK_CHANNEL] = (x == 0)
+        base[..., 5] = x / nx
+        base[..., 6] = y / ny
+        base[..., 7] = z / nz
+        self.base = base
+        cnt = np.zeros((nz + 1, ny + 1, nx + 1))
+        for dz in (0, 1):
+            for dy in (0, 1):
+                for dx in (0, 1):
+                    cnt[dz:dz + nz, dy:dy + ny, dx:dx + nx] += 1.0
+        self.node_count = cnt
+        top = np.zeros((nz + 1, ny + 1, nx + 1), dtype=bool)
+        top[nz] = True
+        self.top = top.reshape(-1)
+
+    def problem(self, k):
+        return problems.cantilever_problem(k, self.nx, self.ny, self.nz, template=self.template)
+
+    def node_log_modulus(self, elem_E):
+        g = np.log(np.asarray(elem_E).reshape(self.nz, self.ny, self.nx) / problems.E_DEFAULT)
+        s = np.zeros((self.nz + 1, self.ny + 1, self.nx + 1))
+        for dz in (0, 1):
+            for dy in (0, 1):
+                for dx in (0, 1):
+                    s[dz:dz + self.nz, dy:dy + self.ny, dx:dx + self.nx] += g
+        return s / self.node_count / problems.CANTILEVER_SIGMA
+
+    def force_scale(self, f):
+        F = np.asarray(f).reshape(-1, 3)
+        return float(np.sqrt(np.mean(F[self.top] ** 2)))
+
+    def features(self, p):
+        """(bf16 bits [nodes * 16], s_f) of problem p."""
+        x = self.base.copy()
+        x[..., 0] = self.node_log_modulus(p.elem_E)
+        s_f = self.force_scale(p.f)
+        x[..., 1:4] = (p.f / s_f).reshape(self.nz + 1, self.ny + 1, self.nx + 1, 3)
+        return bf16_bits(x).reshape(-1), s_f
+
+    def target(self, u, s_f, c_u):
+        t = np.zeros((self.nodes, 4), dtype=np.float32)
+        t[:, :3] = np.asarray(u).reshape(-1, 3) / (s_f * c_u)
+        return t.reshape(-1)
+
+
+def log_default(msg):
+    print("[learned] " + msg, file=sys.stderr, flush=True)
+
+
+def solve_members(ctx, fam, ks, group_size=16, rtol=1e-6, log=log_default, workers=8):
+    """Displacements of members ks from the grouped FP64 Jacobi-PCG, group_size
+    members per launch sequence.  The group's structure is built once and
+    only the moduli change between groups; the next group's problems are
+    generated on host threads while the GPU solves.  Yields (k, problem, u,
+    iterations)."""
+    from concurrent.futures import ThreadPoolExecutor
+    ks = list(ks)
+    chunks = [ks[i:i + group_size] for i in range(0, len(ks), group_size)]
+    if not chunks:
+        return
+    pool = ThreadPoolExecutor(max_workers=workers)
+    futs = [pool.submit(fam.problem, k) for k in chunks[0]]
+    G = None
+    try:
+        for ci, chunk in enumerate(chunks):
+            probs = [f.result() for f in futs]
+            if ci + 1 < len(chunks):
+                futs = [pool.submit(fam.problem, k) for k in chunks[ci + 1]]
+            t0 = time.perf_counter()
+            if G is None or len(G.problems) != len(probs):
+                if G is not None:
+                    G.free()
+                G = ctx.group(probs)
+            else:
+                G.problems = probs
+                G.set_E([p.elem_E for p in probs])
+            G.assemble()
+            G.apply_penalty()
+            x, st, its = G.pcg(np.concatenate([p.f for p in probs]), rtol=rtol)
+            log("solved members %d..%d: %d its max, %.2f s" % (
+                chunk[0], chunk[-1], int(its.max()), time.perf_counter() - t0))
+            for k, p, u, it in zip(chunk, probs, G.split(x), its):
+                yield k, p, u, int(it)
+    finally:
+        pool.shutdown(wait=True)
+        if G is not None:
+            G.free()
+
+
+def cosine_lr(step, total, lr0, lr1):
+    return lr1 + 0.5 * (lr0 - lr1) * (1.0 + np.cos(np.pi * min(step, total) / max(total, 1)))
+
+
+def train(ctx, fam, feats, targets, epochs, batch, lr0=2e-3, lr1=2e-5, seed=0x5EED0008,
+          net=None, log=log_default):
+    """Adam on the GPU over the dataset (feats [N, nodes*16] uint16, targets
+    [N, nodes*4] float32), batch order reshuffled every epoch.  Returns the
+    network and the per-epoch mean losses."""
+    N = feats.shape[0]
+    nb = N // batch
+    if nb < 1:
+        raise ValueError("fewer samples than one batch")
+    if net is None:
+        net = Net(ctx, *fam.grid, batch)
+    in_bytes, t_bytes = fam.nodes * NCH * 2, fam.nodes * 4 * 4
+    d_in = DevBuf(ctx, nb * batch * in_bytes)
+    d_t = DevBuf(ctx, nb * batch * t_bytes)
+    for i in range(nb * batch):
+        d_in.upload(feats[i], i * in_bytes)
+        d_t.upload(targets[i], i * t_bytes)
+    rng = np.random.default_rng(seed)
+    total = epochs * nb
+    step, history = 0, []
+    for ep in range(epochs):
+        t0 = time.perf_counter()
+        losses = []
+        for s in rng.permutation(nb):
+            lr = cosine_lr(step, total, lr0, lr1)
+            losses.append(net.train_step(d_in.at(s * batch * in_bytes), d_t.at(s * batch * t_bytes),
+                                         batch, lr))
+            step += 1
+        history.append(float(np.mean(losses)))
+        log("epoch %d: loss %.4e (%.1f s)" % (ep, history[-1], time.perf_counter() - t0))
+    d_in.free()
+    d_t.free()
+    return net, history
+
+
+def evaluate(ctx, fam, net, c_u, ks, rtol=1e-8, log=log_default):
+    """Zero-start vs warm-start FP64 Jacobi-PCG on members ks (device-resident
+    vectors).  Returns per-member records."""
+    lib = ctx.lib
+    p0 = fam.problem(ks[0])
+    mesh = ctx.upload_mesh(p0.elem_type, p0.ngauss, p0.conn, p0.coords, p0.E, p0.nu,
+                           elem_E=p0.elem_E)
+    K = ctx.symbolic(mesh)
+    n = p0.ndof
+    d_b, d_x = DevBuf(ctx, n * 8), DevBuf(ctx, n * 8)
+    d_in = DevBuf(ctx, fam.nodes * NCH * 2)
+    d_pred = DevBuf(ctx, net.batch_max * fam.nodes * NCH * 4)
+    out = []
+    for k in ks:
+        p = fam.problem(k)
+        eE = np.ascontiguousarray(p.elem_E, dtype=np.float64)
+        ctx._check(lib.gf_mesh_set_E(ctx.handle, mesh.handle, _ptr(eE)))
+        ctx.assemble(mesh, K)
+        ctx.apply_penalty(K, p.fixed_dofs)
+        d_b.upload(np.ascontiguousarray(p.f, dtype=np.float64))
+        st0 = _pcg_dev(ctx, K, d_b, d_x, 0, rtol)
+        u0 = d_x.download(np.float64)
+        feats, s_f = fam.features(p)
+        d_in.upload(feats)
+        t0 = time.perf_counter()
+        net.forward(d_in.ptr, 1, d_pred.ptr)
+        net.to_dofs(d_pred.ptr, d_in.ptr, 0, MASK_CHANNEL, s_f * c_u, d_x.ptr)
+        x0 = d_x.download(np.float64)  # also synchronises
+        t_inf = time.perf_counter() - t0
+        st1 = _pcg_dev(ctx, K, d_b, d_x, 1, rtol)
+        u1 = d_x.download(np.float64)
+        rec = {
+            "k": int(k),
+            "zero_it": int(st0.iterations), "zero_ms": float(st0.ms_device),
+            "zero_true_relres": float(st0.true_relres),
+            "warm_it": int(st1.iterations), "warm_ms": float(st1.ms_device),
+            "warm_true_relres": float(st1.true_relres),
+            "inference_ms": 1e3 * t_inf,
+            "x0_rel_err": float(np.linalg.norm(x0 - u0) / np.linalg.norm(u0)),
+            "solution_rel_diff": float(np.linalg.norm(u1 - u0) / np.linalg.norm(u0)),
+        }
+        out.append(rec)
+        log("member %d: zero %d it %.1f ms | warm %d it %.1f ms + inference %.2f ms | "
+            "x0 err %.2e" % (k, rec["zero_it"], rec["zero_ms"], rec["warm_it"], rec["warm_ms"],
+                             rec["inference_ms"], rec["x0_rel_err"]))
+    return out
+
+
+def _pcg_dev(ctx, K, d_b, d_x, use_x0, rtol):
+    from . import _lib, solve_opts
+    o = solve_opts(device_ptrs=1, use_x0=use_x0, rtol=rtol)
+    st = _lib.SolveStats()
+    ctx._check(ctx.lib.gf_pcg(ctx.handle, K.handle, d_b.ptr, d_x.ptr, 1, C.byref(o), C.byref(st)))
+    return st
+
+
+def summarize(recs):
+    z_it = np.array([r["zero_it"] for r in recs], dtype=float)
+    w_it = np.array([r["warm_it"] for r in recs], dtype=float)
+    z_ms = np.array([r["zero_ms"] for r in recs])
+    w_ms = np.array([r["warm_ms"] + r["inference_ms"] for r in recs])
+    return {
+        "members": len(recs),
+        "zero_iterations_mean": float(z_it.mean()), "warm_iterations_mean": float(w_it.mean()),
+        "iteration_ratio": float(w_it.sum() / z_it.sum()),
+        "zero_ms_mean": float(z_ms.mean()), "warm_plus_inference_ms_mean": float(w_ms.mean()),
+        "time_ratio": float(w_ms.sum() / z_ms.sum()),
+        "max_true_relres_zero": float(max(r["zero_true_relres"] for r in recs)),
+        "max_true_relres_warm": float(max(r["warm_true_relres"] for r in recs)),
+        "x0_rel_err_mean": float(np.mean([r["x0_rel_err"] for r in recs])),
+        "max_solution_rel_diff": float(max(r["solution_rel_diff"] for r in recs)),
+    }
+
+
+def save_weights(path, net, c_u, meta):
+    np.savez(path, params=net.params(), c_u=np.float64(c_u),
+             grid=np.array(net_grid(net), dtype=np.int32),
+             meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
+
+
+def net_grid(net):
+    return (net.nx, net.ny, net.nz)
+
+
+def load_weights(ctx, path, batch_max=1):
+    z = np.load(path, allow_pickle=False)
+    nx, ny, nz = (int(v) for v in z["grid"])
+    net = Net(ctx, nx, ny, nz, batch_max)
+    net.set_params(z["params"])
+    meta = json.loads(bytes(z["meta"]).decode())
+    return net, float(z["c_u"]), meta
+
+
+def main(argv=None):
+    import gpufem_amd as gf
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=2000)
+    ap.add_argument("--holdout", type=int, default=256)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--group", type=int, default=16)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--out", default="gpurun_out/learned_brick8_cantilever.npz")
+    ap.add_argument("--report", default="gpurun_out/learned_report.json")
+    args = ap.parse_args(argv)
+    ctx = gf.Context(0)
+    fam = Family()
+    t0 = time.perf_counter()
+    feats = np.empty((args.train, fam.nodes * NCH), dtype=np.uint16)
+    us, scales, its = [], [], []
+    for i, (k, p, u, it) in enumerate(solve_members(ctx, fam, range(args.train), args.group)):
+        feats[i], s_f = fam.features(p)
+        us.append(u.astype(np.float32))
+        scales.append(s_f)
+        its.append(it)
+    t_data = time.perf_counter() - t0
+    c_u = float(np.sqrt(np.mean([np.mean((u / s) ** 2) for u, s in zip(us, scales)])))
+    targets = np.stack([fam.target(u, s, c_u) for u, s in zip(us, scales)])
+    del us
+    log_default("dataset: %d members in %.1f s, c_u %.4e, mean its %.0f" % (
+        args.train, t_data, c_u, np.mean(its)))
+    t0 = time.perf_counter()
+    net, hist = train(ctx, fam, feats, targets, args.epochs, args.batch, lr0=args.lr)
+    t_train = time.perf_counter() - t0
+    meta = {"family": "brick8_cantilever_64x32x32", "train_members": [0, args.train],
+            "epochs": args.epochs, "batch": args.batch, "loss_history": hist,
+            "train_seconds": t_train, "data_seconds": t_data}
+    save_weights(args.out, net, c_u, meta)
+    eval_net = Net(ctx, *fam.grid, 1)
+    eval_net.set_params(net.params())
+    net.free()
+    ks = list(range(HOLDOUT_BASE, HOLDOUT_BASE + args.holdout))
+    recs = evaluate(ctx, fam, eval_net, c_u, ks)
+    rep = {"summary": summarize(recs), "meta": meta, "members": recs}
+    with open(args.report, "w") as fh:
+        json.dump(rep, fh, indent=1)
+    print(json.dumps(rep["summary"]), flush=True)
+
+
+if __name__ == "__main__":
+    main()

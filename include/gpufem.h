@@ -228,13 +228,16 @@ int gf_pcg(gf_ctx* ctx, gf_ell* m, const double* b, double* x, int32_t nrhs,
 
 /* SpMV schedule of this matrix (the autotuner).  Variants: 0..2 one thread
  * per row with 1/4/8 slots in flight, 3..4 two/four wavefronts per 64-row
- * tile.  force_variant >= 0 fixes a variant; -1 times every variant on this
- * matrix and keeps the fastest (gf_pcg does this on first use).
- * ms_per_variant (may be NULL) receives GF_SPMV_VARIANTS averages.  No
+ * tile, 5/6 one thread per row over the 2x2 / 3x3 node blocks of a matrix
+ * built by gf_ell_symbolic for a 2D / 3D mesh (one node column per block
+ * row, the node's x entries read together).  force_variant >= 0 fixes a
+ * variant; -1 times every variant that fits this matrix and keeps the
+ * fastest (gf_pcg does this on first use).  ms_per_variant (may be NULL)
+ * receives GF_SPMV_VARIANTS averages (-1 for variants that do not fit).  No
  * reference counterpart; it plays the role of SetOclStrategy /
  * SetOclLocalSize (src/SPRmatrix/SPRmatrix.h:110-121) chosen by measurement
  * instead of by hand. */
-#define GF_SPMV_VARIANTS 5
+#define GF_SPMV_VARIANTS 7
 int gf_ell_tune(gf_ctx* ctx, gf_ell* m, int32_t force_variant, double* ms_per_variant,
                 int32_t* chosen);
 
@@ -257,6 +260,9 @@ typedef struct gf_group gf_group;
 int gf_group_create(gf_ctx* ctx, int32_t nmesh, const gf_mesh_desc* meshes, gf_group** out);
 int gf_group_info(const gf_group* g, int64_t* total_dof, int64_t* nnz, int32_t* ncolors);
 int gf_group_assemble(gf_ctx* ctx, gf_group* g, gf_timing* t);
+/* New per-element moduli for every mesh (all meshes' elements concatenated,
+ * host); the structure, coloring and tiles are kept. */
+int gf_group_set_E(gf_ctx* ctx, gf_group* g, const double* elem_E);
 /* dofs: fixed DOFs of every mesh (0-based within its mesh) concatenated;
  * ndofs[m] of them belong to mesh m.  pen_m = coef * max diag of mesh m. */
 int gf_group_apply_penalty(gf_ctx* ctx, gf_group* g, const int32_t* dofs, const int64_t* ndofs,

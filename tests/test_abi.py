@@ -19,7 +19,7 @@ def declared_functions():
     for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
         text = open(h).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        names.update(re.findall(r"\b(gf_[a-z0-9_]+)\s*\(", text))
+        names.update(re.findall(r"\b(gf_[A-Za-z0-9_]+)\s*\(", text))
     return sorted(names)
 
 

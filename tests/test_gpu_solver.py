@@ -232,10 +232,76 @@ def test_spmv_variants_match_oracle(gpu_ctx):
     # one thread per row: same summation order whatever the unroll
     assert np.array_equal(ys[0], ys[1]) and np.array_equal(ys[1], ys[2])
     chosen, ms = gpu_ctx.tune(K)
-    assert 0 <= chosen < 5 and np.all(ms > 0)
+    # an uploaded matrix carries no node structure: the node variants are skipped
+    assert 0 <= chosen < 5 and np.all(ms[:5] > 0) and np.all(ms[5:] == -1)
+    with pytest.raises(gf.GpufemError):
+        gpu_ctx.tune(K, 5)
 
 
-@pytest.mark.parametrize("variant", [0, 2, 3, 4])
+@pytest.mark.parametrize("et,dims,node_variant", [(gf.GF_Q8, (9, 5), 5), (gf.GF_Q4, (13, 7), 5),
+                                                  (gf.GF_BRICK8, (6, 5, 4), 6),
+                                                  (gf.GF_BRICK20, (4, 3, 3), 6)])
+def test_node_block_spmv_is_bitwise_thread_per_row(gpu_ctx, et, dims, node_variant):
+    p, K = assembled(gpu_ctx, et, *dims)
+    x = np.random.default_rng(9).normal(size=p.ndof)
+    gpu_ctx.tune(K, 0)
+    y0, _ = gpu_ctx.spmv(K, x)
+    gpu_ctx.tune(K, node_variant)
+    y1, _ = gpu_ctx.spmv(K, x)
+    assert np.array_equal(y0, y1)
+    with pytest.raises(gf.GpufemError):   # the other dimension's variant does not fit
+        gpu_ctx.tune(K, 11 - node_variant)
+    x1, s1 = gpu_ctx.pcg(K, p.f, schedule=gf.GF_SCHED_KERNELS)
+    gpu_ctx.tune(K, 0)
+    x0, s0 = gpu_ctx.pcg(K, p.f, schedule=gf.GF_SCHED_KERNELS)
+    assert s0.iterations == s1.iterations and np.array_equal(x0, x1)
+    chosen, ms = gpu_ctx.tune(K)
+    assert ms[node_variant] > 0 and ms[11 - node_variant] == -1
+
+
+@pytest.mark.parametrize("et,dims,variant", [(gf.GF_Q8, (9, 5), 7), (gf.GF_Q4, (70, 33), 7),
+                                             (gf.GF_BRICK8, (6, 5, 4), 8),
+                                             (gf.GF_BRICK20, (4, 3, 3), 8),
+                                             (gf.GF_BRICK8, (17, 16, 15), 8)])
+def test_block_sliced_spmv_is_bitwise_thread_per_row(gpu_ctx, et, dims, variant):
+    p, K = assembled(gpu_ctx, et, *dims)
+    x = np.random.default_rng(10).normal(size=p.ndof)
+    gpu_ctx.tune(K, 0)
+    y0, _ = gpu_ctx.spmv(K, x)
+    gpu_ctx.tune(K, variant)
+    y1, _ = gpu_ctx.spmv(K, x)
+    assert np.array_equal(y0, y1)
+    with pytest.raises(gf.GpufemError):
+        gpu_ctx.tune(K, 15 - variant)
+    # the copy follows re-assembly and the penalty (value versions)
+    x1, s1 = gpu_ctx.pcg(K, p.f, schedule=gf.GF_SCHED_KERNELS)
+    gpu_ctx.tune(K, 0)
+    x0, s0 = gpu_ctx.pcg(K, p.f, schedule=gf.GF_SCHED_KERNELS)
+    assert abs(s0.iterations - s1.iterations) <= 1
+    assert np.linalg.norm(x1 - x0) <= 1e-10 * np.linalg.norm(x0)
+    gpu_ctx.tune(K, variant)
+    xm, sm = gpu_ctx.pcg(K, p.f, precision=gf.GF_PREC_MIXED)
+    assert sm.converged and sm.true_relres <= 1e-8
+    chosen, ms = gpu_ctx.tune(K)
+    assert ms[variant] > 0 and ms[15 - variant] == -1
+
+
+def test_block_sliced_copy_tracks_new_values(gpu_ctx):
+    p = P.structured(gf.GF_Q8, 12, 6)
+    mesh = gpu_ctx.upload_mesh(p.elem_type, p.ngauss, p.conn, p.coords, p.E, p.nu)
+    K = gpu_ctx.symbolic(mesh)
+    gpu_ctx.assemble(mesh, K)
+    gpu_ctx.tune(K, 7)
+    x = np.random.default_rng(3).normal(size=p.ndof)
+    ya, _ = gpu_ctx.spmv(K, x)
+    gpu_ctx.apply_penalty(K, p.fixed_dofs)   # changes diagonal values
+    yb, _ = gpu_ctx.spmv(K, x)
+    gpu_ctx.tune(K, 0)
+    yc, _ = gpu_ctx.spmv(K, x)
+    assert not np.array_equal(ya, yb) and np.array_equal(yb, yc)
+
+
+@pytest.mark.parametrize("variant", [0, 2, 3, 4, 5, 7])
 def test_persistent_matches_kernel_schedule(gpu_ctx, variant):
     p, K = assembled(gpu_ctx, gf.GF_Q4, 64, 32)
     gpu_ctx.tune(K, variant)
