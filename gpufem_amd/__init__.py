@@ -238,7 +238,7 @@ class Context:
 
     def tune(self, ell, force_variant=-1):
         """Pick (or force) the SpMV variant; returns (chosen, ms per variant)."""
-        ms = np.zeros(7)
+        ms = np.zeros(_lib.GF_SPMV_VARIANTS)
         chosen = C.c_int32()
         self._check(self.lib.gf_ell_tune(self.handle, ell.handle, force_variant, _ptr(ms),
                                          C.byref(chosen)))

@@ -46,6 +46,14 @@ def _solve(ctx, K, d_b, d_x, nrhs=1, **kw):
     return st
 
 
+def _assembly(p, K, ta, ctx, mesh):
+    # a second numeric assembly (the first one also warms the caches)
+    t2 = ctx.assemble(mesh, K)
+    ctx.apply_penalty(K, p.fixed_dofs)
+    ms = min(ta.ms_device, t2.ms_device)
+    return {"ms": ms, "dof_per_s": p.ndof / (ms * 1e-3), "elements": int(p.conn.shape[0])}
+
+
 def _stats(st):
     return {"iterations": int(st.iterations), "seconds": st.ms_device / 1e3,
             "true_relres": float(st.true_relres)}
@@ -54,7 +62,8 @@ def _stats(st):
 def form_mixed(ctx, n=128):
     p = P.config_brick8_cube(n)
     _log("mixed: %s, %d DOF" % (p.name, p.ndof))
-    mesh, K, _ = _setup(ctx, p)
+    mesh, K, ta = _setup(ctx, p)
+    asm = _assembly(p, K, ta, ctx, mesh)
     d_b, d_x = DevBuf.from_array(ctx, p.f), DevBuf(ctx, p.ndof * 8)
     st64 = _solve(ctx, K, d_b, d_x)
     u64 = d_x.download(np.float64)
@@ -63,7 +72,7 @@ def form_mixed(ctx, n=128):
     u32 = d_x.download(np.float64)
     _log("mixed: mixed %d it (%d refinements, %d fallbacks) %.3f s" % (
         st32.iterations, st32.refinements, st32.fallbacks, st32.ms_device / 1e3))
-    r = {"workload": p.name, "dof": p.ndof, "nnz": K.nnz, "fp64": _stats(st64),
+    r = {"workload": p.name, "dof": p.ndof, "nnz": K.nnz, "assembly": asm, "fp64": _stats(st64),
          "mixed": dict(_stats(st32), refinements=int(st32.refinements),
                        fallbacks=int(st32.fallbacks)),
          "displacement_rel_diff_vs_fp64": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))}
@@ -75,7 +84,8 @@ def form_batched(ctx, n=48, nrhs=32, sequential_sample=4):
     p = P.config_brick20_cube(n)
     F = P.random_top_loads(p, nrhs, P.SEED_BRICK20_RHS)
     _log("batched: %s, %d DOF x %d load cases" % (p.name, p.ndof, nrhs))
-    mesh, K, _ = _setup(ctx, p)
+    mesh, K, ta = _setup(ctx, p)
+    asm = _assembly(p, K, ta, ctx, mesh)
     d_b = DevBuf.from_array(ctx, np.ascontiguousarray(F))
     d_x = DevBuf(ctx, F.nbytes)
     stb = _solve(ctx, K, d_b, d_x, nrhs=nrhs, batched=1)
@@ -93,6 +103,7 @@ def form_batched(ctx, n=48, nrhs=32, sequential_sample=4):
         _log("batched: single load case %d: %d it %.3f s" % (k, st.iterations, st.ms_device / 1e3))
     per_rhs = seq_s / sequential_sample
     r = {"workload": p.name + "_%drhs" % nrhs, "dof": p.ndof, "nnz": K.nnz, "nrhs": nrhs,
+         "assembly": asm,
          "block": _stats(stb),
          "sequential": {"seconds_per_rhs": per_rhs, "iterations": seq_it,
                         "seconds_all_rhs_extrapolated": per_rhs * nrhs,

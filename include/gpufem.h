@@ -230,14 +230,19 @@ int gf_pcg(gf_ctx* ctx, gf_ell* m, const double* b, double* x, int32_t nrhs,
  * per row with 1/4/8 slots in flight, 3..4 two/four wavefronts per 64-row
  * tile, 5/6 one thread per row over the 2x2 / 3x3 node blocks of a matrix
  * built by gf_ell_symbolic for a 2D / 3D mesh (one node column per block
- * row, the node's x entries read together).  force_variant >= 0 fixes a
+ * row, the node's x entries read together), 7/8 one thread per node row of
+ * a block-sliced copy the solver keeps beside the ELL arrays (node rows
+ * sorted by length in windows of 2048, slices of 64, one column index per
+ * 2x2 / 3x3 block; rebuilt from the ELL values after each assembly or
+ * penalty).  Variants 0..2 and 5..8 add each row's entries in slot order
+ * and give bitwise-identical products.  force_variant >= 0 fixes a
  * variant; -1 times every variant that fits this matrix and keeps the
  * fastest (gf_pcg does this on first use).  ms_per_variant (may be NULL)
  * receives GF_SPMV_VARIANTS averages (-1 for variants that do not fit).  No
  * reference counterpart; it plays the role of SetOclStrategy /
  * SetOclLocalSize (src/SPRmatrix/SPRmatrix.h:110-121) chosen by measurement
  * instead of by hand. */
-#define GF_SPMV_VARIANTS 7
+#define GF_SPMV_VARIANTS 9
 int gf_ell_tune(gf_ctx* ctx, gf_ell* m, int32_t force_variant, double* ms_per_variant,
                 int32_t* chosen);
 

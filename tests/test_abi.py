@@ -56,3 +56,13 @@ def test_grid_rejects_bad_arguments():
         gf.Grid(99, 4, 4)
     with pytest.raises(gf.GpufemError):
         gf.Grid(gf.GF_BRICK8, 4, 4, 0)
+
+
+def test_python_constants_match_header():
+    """Sizes the bindings allocate for the library to fill must match the header."""
+    text = open(os.path.join(ROOT, "include", "gpufem.h")).read()
+    defines = dict(re.findall(r"#define (GF_[A-Z0-9_]+) \(?(-?\d+)\)?", text))
+    assert int(defines["GF_SPMV_VARIANTS"]) == _lib.GF_SPMV_VARIANTS
+    for name in ("GF_OK", "GF_ERR_ARG", "GF_ERR_NOCONV", "GF_Q4", "GF_BRICK20", "GF_PREC_MIXED",
+                 "GF_SCHED_PERSISTENT"):
+        assert int(defines[name]) == getattr(_lib, name), name

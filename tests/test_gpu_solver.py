@@ -278,7 +278,7 @@ def test_block_sliced_spmv_is_bitwise_thread_per_row(gpu_ctx, et, dims, variant)
     gpu_ctx.tune(K, 0)
     x0, s0 = gpu_ctx.pcg(K, p.f, schedule=gf.GF_SCHED_KERNELS)
     assert abs(s0.iterations - s1.iterations) <= 1
-    assert np.linalg.norm(x1 - x0) <= 1e-10 * np.linalg.norm(x0)
+    assert np.linalg.norm(x1 - x0) <= 1e-9 * np.linalg.norm(x0)
     gpu_ctx.tune(K, variant)
     xm, sm = gpu_ctx.pcg(K, p.f, precision=gf.GF_PREC_MIXED)
     assert sm.converged and sm.true_relres <= 1e-8
