@@ -128,7 +128,10 @@ def main():
     K = ctx.symbolic(mesh)
     t_sym = time.perf_counter() - t0
     log("symbolic: width %d, nnz %d, %d colors, %.3f s" % (K.width, K.nnz, K.ncolors, t_sym))
-    ctx.assemble(mesh, K)
+    asm_variant, asm_variant_ms = ctx.assemble_tune(mesh, K)
+    log("assembly kernels (ms): %s -> %d" % (" ".join("%.3f" % v for v in asm_variant_ms),
+                                             asm_variant))
+    ctx.apply_penalty(K, p.fixed_dofs)
     variant, variant_ms = ctx.tune(K)
     log("spmv variants (ms): %s -> %d" % (" ".join("%.4f" % v for v in variant_ms), variant))
     sched = {"kernels": gf.GF_SCHED_KERNELS, "graph": gf.GF_SCHED_GRAPH,
@@ -237,6 +240,8 @@ def main():
             "dof_per_s_full": n / (t_asm + t_sym),
             "gflops_dense_convention": asm_flops / t_asm / 1e9,
             "fp64_peak_frac": asm_flops / t_asm / 1e12 / FP64_PEAK_TFLOPS,
+            "kernel_variant": asm_variant,
+            "kernel_variant_ms": [float(v) for v in asm_variant_ms],
         },
         "solve": dict(solves, **{
             "spmv_variant": variant, "spmv_variant_ms": [float(v) for v in variant_ms],

@@ -200,6 +200,14 @@ class Context:
         self._check(self.lib.gf_assemble(self.handle, mesh.handle, ell.handle, C.byref(t)))
         return t
 
+    def assemble_tune(self, mesh, ell, force_variant=-1):
+        """Pick (or force) the assembly kernel; returns (chosen, ms per variant)."""
+        ms = np.zeros(2)
+        chosen = C.c_int32()
+        self._check(self.lib.gf_assemble_tune(self.handle, mesh.handle, ell.handle, force_variant,
+                                              _ptr(ms), C.byref(chosen)))
+        return chosen.value, ms
+
     def apply_penalty(self, ell, dofs, coef=1e4):
         dofs = np.ascontiguousarray(dofs, dtype=np.int32)
         pen = C.c_double()

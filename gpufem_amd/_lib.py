@@ -87,6 +87,7 @@ SIGNATURES = {
     "gf_ell_upload": (C.c_int, [_P, _I64, _I32, _P, _P, _P, C.POINTER(_P)]),
     "gf_ell_download": (C.c_int, [_P, _P, _P, _P, _P]),
     "gf_assemble": (C.c_int, [_P, _P, _P, C.POINTER(Timing)]),
+    "gf_assemble_tune": (C.c_int, [_P, _P, _P, _I32, _P, _P]),
     "gf_apply_penalty": (C.c_int, [_P, _P, _P, _I64, _D, _P]),
     "gf_spmv": (C.c_int, [_P, _P, _P, _P, _I32, C.POINTER(Timing)]),
     "gf_solve_opts_default": (None, [C.POINTER(SolveOpts)]),

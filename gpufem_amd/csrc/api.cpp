@@ -398,17 +398,12 @@ int gf_ell_download(gf_ctx* ctx, const gf_ell* m, double* val, int32_t* col,
 }
 
 // ---------------------------------------------------------------------------
-int gf_assemble(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, gf_timing* t) {
-  if (!ctx || !mesh || !m) return set_error(ctx, GF_ERR_ARG, "gf_assemble: null argument");
-  if (m->ncolors == 0 || m->nelem != mesh->nelem || m->nen != mesh->nen ||
-      m->n != mesh->nnodes * mesh->dim)
-    return set_error(ctx, GF_ERR_ARG, "gf_assemble: matrix was not built for this mesh");
-  int st = select_device(ctx);
-  if (st) return st;
+static int assemble_with(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int variant, gf_timing* t) {
   const double h0 = now_ms();
   int* d_bad = reinterpret_cast<int*>(static_cast<char*>(ctx->d_state) + 128);
   GF_HIP_CHECK(ctx, hipMemsetAsync(d_bad, 0, sizeof(int), ctx->stream));
-  if ((st = record_start(ctx))) return st;
+  int st = record_start(ctx);
+  if (st) return st;
   const size_t slots = size_t(m->n) * size_t(m->width);
   GF_HIP_CHECK(ctx, hipMemsetAsync(m->d_val, 0, slots * sizeof(double), ctx->stream));
   AssemblyArgs a;
@@ -423,6 +418,7 @@ int gf_assemble(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, gf_timing* t) {
   a.val = m->d_val;
   a.n = m->n;
   a.bad_jac = d_bad;
+  a.variant = variant;
   int64_t launches = 1;
   for (int c = 0; c < m->ncolors; ++c) {
     a.elems = m->d_color_elems + m->color_ptr[size_t(c)];
@@ -445,6 +441,52 @@ int gf_assemble(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, gf_timing* t) {
   if (bad)
     return set_error(ctx, GF_ERR_DEGENERATE, "gf_assemble: an element has detJ <= 0");
   return GF_OK;
+}
+
+static int check_assembly_args(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, const char* fn) {
+  if (!ctx || !mesh || !m) return set_error(ctx, GF_ERR_ARG, std::string(fn) + ": null argument");
+  if (m->ncolors == 0 || m->nelem != mesh->nelem || m->nen != mesh->nen ||
+      m->n != mesh->nnodes * mesh->dim)
+    return set_error(ctx, GF_ERR_ARG, std::string(fn) + ": matrix was not built for this mesh");
+  return select_device(ctx);
+}
+
+int gf_assemble_tune(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int32_t force,
+                     double* ms_per_variant, int32_t* chosen) {
+  int st = check_assembly_args(ctx, mesh, m, "gf_assemble_tune");
+  if (st) return st;
+  if (force > 1) return set_error(ctx, GF_ERR_ARG, "gf_assemble_tune: variant must be 0 or 1");
+  if (force >= 0) {
+    m->asm_variant = force;
+  } else {
+    double best = 1e300;
+    int best_v = 0;
+    for (int v = 0; v < 2; ++v) {
+      gf_timing t1{}, t2{};
+      if ((st = assemble_with(ctx, mesh, m, v, &t1)) || (st = assemble_with(ctx, mesh, m, v, &t2)))
+        return st;
+      const double ms = std::min(t1.ms_device, t2.ms_device);
+      if (ms_per_variant) ms_per_variant[v] = ms;
+      if (ms < best) {
+        best = ms;
+        best_v = v;
+      }
+    }
+    m->asm_variant = best_v;
+  }
+  if (chosen) *chosen = m->asm_variant;
+  return GF_OK;
+}
+
+int gf_assemble(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, gf_timing* t) {
+  int st = check_assembly_args(ctx, mesh, m, "gf_assemble");
+  if (st) return st;
+  if (m->asm_variant < 0) {
+    // small meshes: not worth timing; large ones pick the faster kernel once
+    if (m->nelem < 65536) m->asm_variant = 0;
+    else if ((st = gf_assemble_tune(ctx, mesh, m, -1, nullptr, nullptr))) return st;
+  }
+  return assemble_with(ctx, mesh, m, m->asm_variant, t);
 }
 
 int gf_apply_penalty(gf_ctx* ctx, gf_ell* m, const int32_t* dofs, int64_t nd, double coef,

@@ -93,6 +93,9 @@ struct gf_ell {
   std::vector<int32_t> h_elem_color;  // [nelem], 1-based colors
   int64_t nelem = 0;
   int nen = 0;
+  // assembly kernel variant (0 element-major, 1 pair-major; -1: time both
+  // on the first assembly of a large mesh)
+  int asm_variant = -1;
   // mixed precision copy (lazily built)
   float* d_val32 = nullptr;
   double* d_dinv = nullptr;

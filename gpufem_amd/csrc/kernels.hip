@@ -178,6 +178,117 @@ __global__ __launch_bounds__(256) void k_assemble(KArgs a) {
   }
 }
 
+// Pair-major variant: EPB elements per workgroup and consecutive threads on
+// the same node pair of consecutive elements of the color, so the
+// read-modify-write of one pair touches neighbouring rows together; the LDS
+// tables are element-fastest ([g][i][node][element]) so those threads also
+// read consecutive words.  Same arithmetic, same order: bitwise equal to
+// k_assemble.
+template <int DIM, int NEN, int NGP, int EPB>
+__global__ __launch_bounds__(256) void k_assemble_pm(KArgs a) {
+  constexpr int PAIRS = NEN * NEN;
+  __shared__ double s_dndx[NGP][DIM][NEN][EPB];
+  __shared__ double s_sc[NGP][EPB];
+  __shared__ double s_x[DIM][NEN][EPB];
+  __shared__ int32_t s_node[NEN][EPB];
+  __shared__ int32_t s_elem[EPB];
+  const int tid = threadIdx.x;
+  const int64_t base = int64_t(blockIdx.x) * EPB;
+  const int nel = int(a.nce - base < EPB ? a.nce - base : EPB);
+
+  for (int t = tid; t < EPB * NEN; t += 256) {
+    const int el = t % EPB, an = t / EPB;
+    if (el < nel) {
+      const int32_t e = a.elems[base + el];
+      if (an == 0) s_elem[el] = e;
+      const int32_t node = a.conn[int64_t(e) * NEN + an] - 1;
+      s_node[an][el] = node;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) s_x[d][an][el] = a.coords[int64_t(node) * DIM + d];
+    }
+  }
+  __syncthreads();
+
+  for (int t = tid; t < EPB * NGP; t += 256) {
+    const int el = t % EPB, g = t / EPB;
+    if (el >= nel) continue;
+    const double* dN = c_tab + a.tab + g * DIM * NEN;
+    double J[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) {
+        double acc = 0.0;
+#pragma unroll
+        for (int n = 0; n < NEN; ++n) acc += dN[i * NEN + n] * s_x[j][n][el];
+        J[i][j] = acc;
+      }
+    double Ji[DIM][DIM];
+    const double det = det_inv<DIM>(J, Ji);
+    if (!(det > 0.0)) *a.bad = 1;
+#pragma unroll
+    for (int n = 0; n < NEN; ++n)
+#pragma unroll
+      for (int i = 0; i < DIM; ++i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) acc += Ji[i][k] * dN[k * NEN + n];
+        s_dndx[g][i][n][el] = acc;
+      }
+    const double w = c_tab[a.tab + NGP * DIM * NEN + g];
+    const double Ee = a.elemE ? a.elemE[s_elem[el]] : a.E;
+    s_sc[g][el] = w * det * Ee;
+  }
+  __syncthreads();
+
+  for (int t = tid; t < EPB * PAIRS; t += 256) {
+    const int el = t % EPB, p = t / EPB;
+    if (el >= nel) continue;
+    const int an = p / NEN, bn = p % NEN;
+    double k[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) k[i][j] = 0.0;
+    for (int g = 0; g < NGP; ++g) {
+      const double sc = s_sc[g][el];
+      double ga[DIM], gb[DIM];
+#pragma unroll
+      for (int i = 0; i < DIM; ++i) {
+        ga[i] = s_dndx[g][i][an][el];
+        gb[i] = s_dndx[g][i][bn][el];
+      }
+      if constexpr (DIM == 2) {
+        k[0][0] += sc * (a.c0 * ga[0] * gb[0] + a.c2 * ga[1] * gb[1]);
+        k[0][1] += sc * (a.c1 * ga[0] * gb[1] + a.c2 * ga[1] * gb[0]);
+        k[1][0] += sc * (a.c1 * ga[1] * gb[0] + a.c2 * ga[0] * gb[1]);
+        k[1][1] += sc * (a.c0 * ga[1] * gb[1] + a.c2 * ga[0] * gb[0]);
+      } else {
+        const double dot = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            if (i == j)
+              k[i][i] += sc * (a.c0 * ga[i] * gb[i] + a.c2 * (dot - ga[i] * gb[i]));
+            else
+              k[i][j] += sc * (a.c1 * ga[i] * gb[j] + a.c2 * ga[j] * gb[i]);
+          }
+      }
+    }
+    const int32_t e = s_elem[el];
+    const int64_t slot = a.slot[(int64_t(e) * NEN + an) * NEN + bn];
+    const int64_t row0 = int64_t(s_node[an][el]) * DIM;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) {
+        double* v = a.val + (row0 + i) + (slot * DIM + j) * a.n;
+        *v += k[i][j];
+      }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // ELL expansion: row dim*v+c gets columns dim*w+c' for w in adj(v).
 // ---------------------------------------------------------------------------
@@ -271,8 +382,24 @@ inline int grid_for(int64_t work, int threads, int cap) {
   return int(b);
 }
 
+// elements per workgroup of the pair-major kernel: LDS of the dN/dx tables
+// about 48 KB or less
 template <int DIM, int NEN, int NGP>
-hipError_t launch_asm(hipStream_t s, const KArgs& k) {
+constexpr int pm_epb() {
+  constexpr int per = NGP * DIM * NEN * 8;
+  return per * 32 <= 50 * 1024 ? 32 : (per * 16 <= 64 * 1024 ? 16 : (per * 8 <= 64 * 1024 ? 8 : 4));
+}
+
+template <int DIM, int NEN, int NGP>
+hipError_t launch_asm(hipStream_t s, const KArgs& k, int variant) {
+  if (variant == 1) {
+    constexpr int EPB = pm_epb<DIM, NEN, NGP>();
+    const int64_t blocks = (k.nce + EPB - 1) / EPB;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_assemble_pm<DIM, NEN, NGP, EPB>), dim3(unsigned(blocks)), dim3(256), 0,
+                       s, k);
+    return hipGetLastError();
+  }
   constexpr int PAIRS = NEN * NEN;
   constexpr int EPB = PAIRS >= 256 ? 1 : 256 / PAIRS;
   const int64_t blocks = (k.nce + EPB - 1) / EPB;
@@ -337,11 +464,12 @@ hipError_t launch_assembly_color(hipStream_t s, const AssemblyArgs& a) {
   k.bad = a.bad_jac;
   k.tab = h_off[tab_block(a.type, a.ngauss)];
   const bool g3 = a.ngauss == 3;
+  const int v = a.variant;
   switch (a.type) {
-    case GF_Q4: return g3 ? launch_asm<2, 4, 9>(s, k) : launch_asm<2, 4, 4>(s, k);
-    case GF_Q8: return g3 ? launch_asm<2, 8, 9>(s, k) : launch_asm<2, 8, 4>(s, k);
-    case GF_BRICK8: return g3 ? launch_asm<3, 8, 27>(s, k) : launch_asm<3, 8, 8>(s, k);
-    case GF_BRICK20: return g3 ? launch_asm<3, 20, 27>(s, k) : launch_asm<3, 20, 8>(s, k);
+    case GF_Q4: return g3 ? launch_asm<2, 4, 9>(s, k, v) : launch_asm<2, 4, 4>(s, k, v);
+    case GF_Q8: return g3 ? launch_asm<2, 8, 9>(s, k, v) : launch_asm<2, 8, 4>(s, k, v);
+    case GF_BRICK8: return g3 ? launch_asm<3, 8, 27>(s, k, v) : launch_asm<3, 8, 8>(s, k, v);
+    case GF_BRICK20: return g3 ? launch_asm<3, 20, 27>(s, k, v) : launch_asm<3, 20, 8>(s, k, v);
     default: return hipErrorInvalidValue;
   }
 }

@@ -163,6 +163,15 @@ int gf_ell_download(gf_ctx* ctx, const gf_ell* m, double* val, int32_t* col,
 /* Numeric assembly: Ke by Gauss quadrature fused with the colored,
  * atomic-free scatter (one launch per color).  Values are zeroed first. */
 int gf_assemble(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, gf_timing* t);
+/* Assembly kernel of this matrix: 0 = one thread per (element, node pair)
+ * with the element outermost; 1 = pair-major, consecutive threads on the same
+ * node pair of consecutive elements of a color (neighbouring rows in the
+ * scatter).  Both add the same terms in the same order (bitwise-identical
+ * values).  force_variant 0/1 fixes it; -1 assembles twice with each and
+ * keeps the faster (gf_assemble does this on its first call for meshes of
+ * 65536 elements or more).  ms_per_variant (may be NULL): 2 averages. */
+int gf_assemble_tune(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int32_t force_variant,
+                     double* ms_per_variant, int32_t* chosen);
 
 /* Penalty constraint (FEM::ApplyConstraint PEN): pen = coef * max_i K_ii,
  * then K_dd <- pen * K_dd for each listed DOF (0-based; a DOF listed k

@@ -78,6 +78,25 @@ def test_assembly_deterministic(gpu_ctx):
     assert np.array_equal(v1, v2)
 
 
+@pytest.mark.parametrize("t,nx,ny,nz,ng", [(gf.GF_Q4, 20, 9, 0, 2), (gf.GF_Q8, 11, 6, 0, 3),
+                                           (gf.GF_BRICK8, 7, 5, 4, 2), (gf.GF_BRICK8, 4, 3, 3, 3),
+                                           (gf.GF_BRICK20, 4, 3, 3, 3),
+                                           (gf.GF_BRICK20, 3, 3, 2, 2)])
+def test_pair_major_assembly_is_bitwise_element_major(gpu_ctx, t, nx, ny, nz, ng):
+    g, conn, coords, mesh, K, Ko = build_both(gpu_ctx, t, nx, ny, nz, ng)
+    gpu_ctx.assemble_tune(mesh, K, 0)
+    gpu_ctx.assemble(mesh, K)
+    v0, c0, r0 = gpu_ctx.download(K)
+    gpu_ctx.assemble_tune(mesh, K, 1)
+    gpu_ctx.assemble(mesh, K)
+    v1, c1, r1 = gpu_ctx.download(K)
+    assert np.array_equal(v0, v1) and np.array_equal(c0, c1)
+    chosen, ms = gpu_ctx.assemble_tune(mesh, K)
+    assert chosen in (0, 1) and np.all(ms > 0)
+    with pytest.raises(gf.GpufemError):
+        gpu_ctx.assemble_tune(mesh, K, 2)
+
+
 def test_penalty_parity(gpu_ctx):
     p = P.config_q4_plate()
     mesh = gpu_ctx.upload_mesh(p.elem_type, p.ngauss, p.conn, p.coords, p.E, p.nu)
