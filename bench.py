@@ -91,6 +91,27 @@ def cpu_baseline(p, gpu_iters, budget_s):
     }
 
 
+COPY_BYTES = 1 << 30
+
+
+def probe(ctx, lib, p, K, variant, nrep=20):
+    """nrep launches of the solver's SpMV kernel, then nrep of a plain copy of
+    COPY_BYTES (read COPY_BYTES, write COPY_BYTES): the copy calibrates the
+    per-dispatch byte counters of the same run."""
+    import gpufem_amd as gf
+    ms_spmv = C.c_double()
+    gf.check(lib.gf_profile_spmv(ctx.handle, K.handle, nrep, C.byref(ms_spmv)), ctx.handle)
+    src, dst = gf.DevBuf(ctx, COPY_BYTES), gf.DevBuf(ctx, COPY_BYTES)
+    ms_copy = C.c_double()
+    gf.check(lib.gf_stream_copy(ctx.handle, dst.ptr, src.ptr, COPY_BYTES, nrep,
+                                C.byref(ms_copy)), ctx.handle)
+    print(json.dumps({"workload": p.name, "spmv_variant": variant, "nnz": K.nnz, "n": p.ndof,
+                      "spmv_ms": ms_spmv.value, "spmv_launches": nrep + 1,
+                      "copy_bytes_each_way": COPY_BYTES, "copy_ms": ms_copy.value,
+                      "copy_launches": nrep + 1,
+                      "copy_gbs": 2 * COPY_BYTES / (ms_copy.value * 1e-3) / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +129,9 @@ def main():
                          "workloads (mixed, batched, grouped, learned) or 'none'")
     ap.add_argument("--weights", default=os.path.join(ROOT, "gpufem_amd", "weights",
                                                       "brick8_cantilever_64x32x32.npz"))
+    ap.add_argument("--probe", action="store_true",
+                    help="only launch the solver's SpMV kernel and a plain copy kernel of known "
+                         "size (for hardware-counter runs); prints one JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: --gpus %d is not supported (the path would "
@@ -136,6 +160,9 @@ def main():
     log("spmv variants (ms): %s -> %d" % (" ".join("%.4f" % v for v in variant_ms), variant))
     sched = {"kernels": gf.GF_SCHED_KERNELS, "graph": gf.GF_SCHED_GRAPH,
              "persistent": gf.GF_SCHED_PERSISTENT}[args.schedule]
+    if args.probe:
+        probe(ctx, lib, p, K, variant)
+        return
 
     # inputs resident in HBM
     vb = n * 8
@@ -210,6 +237,16 @@ def main():
     gf.check(lib.gf_profile_spmv(ctx.handle, K.handle, 50, C.byref(ms_spmv)), ctx.handle)
     spmv_bytes = 12 * K.nnz + 16 * n
     achieved = spmv_bytes / (ms_spmv.value * 1e-3) / 1e9
+    # attainable streaming rate on this GPU: a plain copy kernel
+    cs, cd = gf.DevBuf(ctx, COPY_BYTES), gf.DevBuf(ctx, COPY_BYTES)
+    ms_copy = C.c_double()
+    gf.check(lib.gf_stream_copy(ctx.handle, cd.ptr, cs.ptr, COPY_BYTES, 20, C.byref(ms_copy)),
+             ctx.handle)
+    copy_gbs = 2 * COPY_BYTES / (ms_copy.value * 1e-3) / 1e9
+    cs.free()
+    cd.free()
+    log("spmv %.4f ms (%.0f GB/s algorithmic); copy kernel %.0f GB/s" % (
+        ms_spmv.value, achieved, copy_gbs))
     ne = p.grid.nelem
     ngp = p.ngauss ** p.dim
     asm_flops = ke_flops_per_element(p.dim, p.grid.nen, ngp) * ne
@@ -256,6 +293,8 @@ def main():
             "traffic": None,
             "algorithmic_bytes_per_launch": spmv_bytes,
             "ms_per_launch": ms_spmv.value,
+            "copy_kernel_gbs": copy_gbs,
+            "frac_of_copy": achieved / copy_gbs,
         },
         "cpu_baseline": None,
     }

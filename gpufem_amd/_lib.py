@@ -94,6 +94,7 @@ SIGNATURES = {
     "gf_pcg": (C.c_int, [_P, _P, _P, _P, _I32, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "gf_ell_tune": (C.c_int, [_P, _P, _I32, _P, _P]),
     "gf_profile_spmv": (C.c_int, [_P, _P, _I32, C.POINTER(C.c_double)]),
+    "gf_stream_copy": (C.c_int, [_P, _P, _P, _I64, _I32, C.POINTER(C.c_double)]),
     "gf_group_create": (C.c_int, [_P, _I32, C.POINTER(MeshDesc), C.POINTER(_P)]),
     "gf_group_info": (C.c_int, [_P, _P, _P, _P]),
     "gf_group_assemble": (C.c_int, [_P, _P, C.POINTER(Timing)]),

@@ -575,6 +575,23 @@ int gf_spmv(gf_ctx* ctx, const gf_ell* m, const double* x, double* y, int32_t nr
   return GF_OK;
 }
 
+int gf_stream_copy(gf_ctx* ctx, void* dst, const void* src, int64_t bytes, int32_t nrep,
+                   double* ms_avg) {
+  if (!ctx || !dst || !src || bytes <= 0 || nrep < 1)
+    return set_error(ctx, GF_ERR_ARG, "gf_stream_copy: bad argument");
+  int st = select_device(ctx);
+  if (st) return st;
+  GF_HIP_CHECK(ctx, launch_stream_copy(ctx->stream, src, dst, bytes));  // warm
+  GF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
+  for (int i = 0; i < nrep; ++i) GF_HIP_CHECK(ctx, launch_stream_copy(ctx->stream, src, dst, bytes));
+  GF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
+  GF_HIP_CHECK(ctx, hipEventSynchronize(ctx->ev_stop));
+  float ms = 0.f;
+  GF_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+  if (ms_avg) *ms_avg = double(ms) / nrep;
+  return GF_OK;
+}
+
 int gf_dev_alloc(gf_ctx* ctx, int64_t bytes, void** dptr) {
   if (!ctx || !dptr || bytes < 0) return set_error(ctx, GF_ERR_ARG, "gf_dev_alloc: bad argument");
   int st = select_device(ctx);

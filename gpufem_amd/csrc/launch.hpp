@@ -90,6 +90,8 @@ int spmv_blocks(int64_t n);
 hipError_t launch_spmv(hipStream_t s, int variant, int64_t n, const double* val,
                        const int32_t* col, const int32_t* rownnz, const double* x, double* y,
                        int nrhs, const BsellView* bs = nullptr);
+// plain copy of `bytes` (a multiple of 16, 16-byte aligned pointers)
+hipError_t launch_stream_copy(hipStream_t s, const void* src, void* dst, int64_t bytes);
 hipError_t launch_jacobi_inv(hipStream_t s, int64_t n, const double* val,
                              const int32_t* diag, double* dinv);
 

@@ -107,6 +107,14 @@ __global__ void k_build_coln(int64_t n, int32_t nbw, int d, const int32_t* col, 
   }
 }
 
+// plain streaming copy, 16 bytes per thread per step (the bandwidth probe)
+typedef double v2d __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(kNT) void k_stream_copy(const v2d* __restrict__ src,
+                                                     v2d* __restrict__ dst, int64_t n16) {
+  for (int64_t i = int64_t(blockIdx.x) * kNT + threadIdx.x; i < n16; i += int64_t(gridDim.x) * kNT)
+    dst[i] = __builtin_nontemporal_load(src + i);
+}
+
 __global__ void k_jacobi_inv(int64_t n, const double* val, const int32_t* diag, double* dinv) {
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
@@ -414,6 +422,15 @@ hipError_t launch_build_coln(hipStream_t s, int64_t n, int32_t width, int d, con
   if (nb > 8192) nb = 8192;
   if (nb < 1) nb = 1;
   hipLaunchKernelGGL(k_build_coln, dim3(unsigned(nb)), dim3(kNT), 0, s, n, width / d, d, col, coln);
+  return hipGetLastError();
+}
+
+hipError_t launch_stream_copy(hipStream_t s, const void* src, void* dst, int64_t bytes) {
+  if ((bytes & 15) || (reinterpret_cast<uintptr_t>(src) & 15) || (reinterpret_cast<uintptr_t>(dst) & 15))
+    return hipErrorInvalidValue;
+  const int64_t n16 = bytes / 16;
+  hipLaunchKernelGGL(k_stream_copy, dim3(grid_for(n16, kNT, 8192)), dim3(kNT), 0, s,
+                     static_cast<const v2d*>(src), static_cast<v2d*>(dst), n16);
   return hipGetLastError();
 }
 

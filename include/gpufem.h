@@ -261,6 +261,13 @@ int gf_ell_tune(gf_ctx* ctx, gf_ell* m, int32_t force_variant, double* ms_per_va
  * it mirrors the per-kernel event timing of OCLwrapper
  * (src/OpenCL/OCLwrapper.cc:400-434) for the roofline figure. */
 int gf_profile_spmv(gf_ctx* ctx, gf_ell* m, int32_t nrep, double* ms_avg);
+/* Bandwidth probe: nrep launches of a plain streaming copy kernel of `bytes`
+ * (multiple of 16; 16-byte aligned device pointers) on the context's stream;
+ * *ms_avg = average launch time.  Measurement aid with no reference
+ * counterpart: the copy's achieved rate is the attainable HBM ceiling, and
+ * its counter readings calibrate hardware byte counters. */
+int gf_stream_copy(gf_ctx* ctx, void* dst, const void* src, int64_t bytes, int32_t nrep,
+                   double* ms_avg);
 
 /* ======================================================================== */
 /* Grouped batch: many independent meshes of one element type (same Gauss   */
