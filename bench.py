@@ -124,7 +124,7 @@ def main():
                     choices=["kernels", "graph", "persistent"])
     ap.add_argument("--compare", default="all", choices=["all", "none"],
                     help="also time the other FP64 schedules and the mixed solve")
-    ap.add_argument("--forms", default="mixed,batched,grouped,learned",
+    ap.add_argument("--forms", default="mixed,batched,grouped,learned,combined",
                     help="comma list of forms timed against plain FP64 on their own "
                          "workloads (mixed, batched, grouped, learned) or 'none'")
     ap.add_argument("--weights", default=os.path.join(ROOT, "gpufem_amd", "weights",

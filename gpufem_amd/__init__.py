@@ -326,11 +326,16 @@ class Group:
         self.ctx._check(self.lib.gf_group_apply_penalty(self.ctx.handle, self.handle, _ptr(dofs),
                                                         _ptr(nd), coef))
 
-    def pcg(self, b=None, raise_on_fail=True, **opts):
+    def pcg(self, b=None, raise_on_fail=True, x0=None, **opts):
+        """Solve every mesh; b and x0 (optional initial guess) concatenated."""
         if b is None:
             b = np.concatenate([p.f for p in self.problems])
         b = np.ascontiguousarray(b, dtype=np.float64)
-        x = np.zeros_like(b)
+        if x0 is not None:
+            x = np.array(x0, dtype=np.float64, copy=True).reshape(b.shape)
+            opts.setdefault("use_x0", 1)
+        else:
+            x = np.zeros_like(b)
         o = solve_opts(**opts)
         st = _lib.SolveStats()
         its = np.zeros(len(self.problems), dtype=np.int32)

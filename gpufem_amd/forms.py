@@ -164,6 +164,83 @@ def form_learned(ctx, weights, members=16):
     return s
 
 
+def form_combined(ctx, weights, members=64, group=16):
+    """Learned warm start + grouped batch together on held-out cantilevers,
+    against plain FP64 PCG from zero member after member (same meshes, same
+    true-residual stopping test; assembly counted on both sides)."""
+    from . import learned as L
+    if not os.path.exists(weights):
+        return {"skipped": "no trained weights at %s" % weights}
+    net, c_u, meta = L.load_weights(ctx, weights, batch_max=group)
+    fam = L.Family()
+    ks = list(range(L.HOLDOUT_BASE, L.HOLDOUT_BASE + members))
+    probs = [fam.problem(k) for k in ks]
+    _log("combined: %d held-out cantilevers, groups of %d" % (members, group))
+    # plain: one member after another
+    p0 = probs[0]
+    mesh = ctx.upload_mesh(p0.elem_type, p0.ngauss, p0.conn, p0.coords, p0.E, p0.nu,
+                           elem_E=p0.elem_E)
+    K = ctx.symbolic(mesh)
+    d_b, d_x = DevBuf(ctx, p0.ndof * 8), DevBuf(ctx, p0.ndof * 8)
+    plain_s, plain_it, u_plain = 0.0, [], []
+    for p in probs:
+        ctx._check(ctx.lib.gf_mesh_set_E(ctx.handle, mesh.handle,
+                                         np.ascontiguousarray(p.elem_E).ctypes.data_as(C.c_void_p)))
+        ta = ctx.assemble(mesh, K)
+        ctx.apply_penalty(K, p.fixed_dofs)
+        d_b.upload(np.ascontiguousarray(p.f))
+        st = _solve(ctx, K, d_b, d_x)
+        plain_s += (ta.ms_device + st.ms_device) / 1e3
+        plain_it.append(int(st.iterations))
+        u_plain.append(d_x.download(np.float64))
+    _log("combined: plain %.3f s, %.0f it mean" % (plain_s, np.mean(plain_it)))
+    # combined: groups, x0 from the network (batched inference)
+    nodes = fam.nodes
+    d_in = DevBuf(ctx, group * nodes * L.NCH * 2)
+    d_pred = DevBuf(ctx, group * nodes * L.NCH * 4)
+    comb_s, comb_it, worst, max_res = 0.0, [], 0.0, 0.0
+    G = None
+    for g0 in range(0, members, group):
+        chunk = probs[g0:g0 + group]
+        if G is None or len(G.problems) != len(chunk):
+            G = ctx.group(chunk)
+        else:
+            G.problems = chunk
+            G.set_E([p.elem_E for p in chunk])
+        ta = G.assemble()
+        G.apply_penalty()
+        scales = []
+        for i, p in enumerate(chunk):
+            f, s_f = fam.features(p)
+            d_in.upload(f, i * nodes * L.NCH * 2)
+            scales.append(s_f * c_u)
+        t0 = time.perf_counter()
+        net.forward(d_in.ptr, len(chunk), d_pred.ptr)
+        pred = d_pred.download(np.float32, len(chunk) * nodes * L.NCH).reshape(len(chunk), nodes,
+                                                                             L.NCH)
+        t_inf = time.perf_counter() - t0
+        mask = fam.base[..., L.MASK_CHANNEL].reshape(-1) > 0.5
+        x0 = []
+        for i in range(len(chunk)):
+            xi = scales[i] * pred[i, :, :3].astype(np.float64)
+            xi[mask] = 0.0
+            x0.append(xi.reshape(-1))
+        x, st, its = G.pcg(np.concatenate([p.f for p in chunk]), x0=np.concatenate(x0), rtol=RTOL)
+        comb_s += (ta.ms_device + st.ms_device) / 1e3 + t_inf
+        comb_it += [int(v) for v in its]
+        max_res = max(max_res, float(st.true_relres))
+        for i, xi in enumerate(G.split(x)):
+            u = u_plain[g0 + i]
+            worst = max(worst, float(np.linalg.norm(xi - u) / np.linalg.norm(u)))
+    _log("combined: grouped + warm start %.3f s, %.0f it mean" % (comb_s, np.mean(comb_it)))
+    return {"workload": "brick8_cantilever_64x32x32_x%d" % members, "members": members,
+            "group": group,
+            "plain_fp64": {"seconds": plain_s, "iterations_mean": float(np.mean(plain_it))},
+            "grouped_learned": {"seconds": comb_s, "iterations_mean": float(np.mean(comb_it)),
+                                "max_true_relres": max_res},
+            "max_rel_diff_vs_plain": worst, "speedup": plain_s / comb_s}
+
+
 FORMS = {"mixed": form_mixed, "batched": form_batched, "grouped": form_grouped}
 
 
@@ -172,7 +249,12 @@ def run(ctx, names, weights):
     for name in names:
         t0 = time.perf_counter()
         try:
-            out[name] = form_learned(ctx, weights) if name == "learned" else FORMS[name](ctx)
+            if name == "learned":
+                out[name] = form_learned(ctx, weights)
+            elif name == "combined":
+                out[name] = form_combined(ctx, weights)
+            else:
+                out[name] = FORMS[name](ctx)
         except Exception as e:  # report, keep the other forms
             out[name] = {"error": "%s: %s" % (type(e).__name__, e)}
         out[name]["wall_s"] = time.perf_counter() - t0
