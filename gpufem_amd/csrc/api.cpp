@@ -455,7 +455,8 @@ int gf_assemble_tune(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int32_t force,
                      double* ms_per_variant, int32_t* chosen) {
   int st = check_assembly_args(ctx, mesh, m, "gf_assemble_tune");
   if (st) return st;
-  if (force > 1) return set_error(ctx, GF_ERR_ARG, "gf_assemble_tune: variant must be 0 or 1");
+  if (force > 2 || (force == 2 && mesh->dim != 3))
+    return set_error(ctx, GF_ERR_ARG, "gf_assemble_tune: variant must be 0, 1 or (3D) 2");
   if (force >= 0) {
     m->asm_variant = force;
   } else {

@@ -289,6 +289,135 @@ __global__ __launch_bounds__(256) void k_assemble_pm(KArgs a) {
   }
 }
 
+// FP64 matrix-core variant for the 3D elements: one wavefront per element
+// computes Ke = sum_g B_g^T (sc_g D B_g) as a (3 nen) x (3 nen) GEMM with
+// K = 6 ngp on v_mfma_f64_16x16x4f64.  Lane l supplies A[l%16][l/16] and
+// B[l/16][l%16] and receives D[4(l/16)+r][l%16] (r = 0..3).  The operands
+// are generated from the dN/dx tables in LDS (B_g has the 6-strain sparsity
+// pattern, D is isotropic); the result is scattered like k_assemble.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <int NEN>
+__device__ inline double strain_b(const double* ga, int s, int c) {
+  // B_g[s][3a + c] for node a with dN/dx = ga[0..2]; strains xx yy zz xy yz zx
+  switch (s) {
+    case 0: return c == 0 ? ga[0] : 0.0;
+    case 1: return c == 1 ? ga[1] : 0.0;
+    case 2: return c == 2 ? ga[2] : 0.0;
+    case 3: return c == 0 ? ga[1] : (c == 1 ? ga[0] : 0.0);
+    case 4: return c == 1 ? ga[2] : (c == 2 ? ga[1] : 0.0);
+    default: return c == 0 ? ga[2] : (c == 2 ? ga[0] : 0.0);
+  }
+}
+
+template <int NEN, int NGP>
+__global__ __launch_bounds__(256) void k_assemble_mfma(KArgs a) {
+  constexpr int NE = 3 * NEN, NT = (NE + 15) / 16, KS = (6 * NGP + 3) / 4;
+  constexpr int EPB = 4;  // one wavefront per element
+  __shared__ double s_dndx[EPB][NGP][3][NEN];
+  __shared__ double s_sc[EPB][NGP];
+  __shared__ double s_x[EPB][3][NEN];
+  __shared__ int32_t s_node[EPB][NEN];
+  __shared__ int32_t s_elem[EPB];
+  const int tid = threadIdx.x;
+  const int64_t base = int64_t(blockIdx.x) * EPB;
+  for (int t = tid; t < EPB * NEN; t += 256) {
+    const int el = t / NEN, an = t % NEN;
+    if (base + el < a.nce) {
+      const int32_t e = a.elems[base + el];
+      if (an == 0) s_elem[el] = e;
+      const int32_t node = a.conn[int64_t(e) * NEN + an] - 1;
+      s_node[el][an] = node;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) s_x[el][d][an] = a.coords[int64_t(node) * 3 + d];
+    }
+  }
+  __syncthreads();
+  for (int t = tid; t < EPB * NGP; t += 256) {
+    const int el = t / NGP, g = t % NGP;
+    if (base + el >= a.nce) continue;
+    const double* dN = c_tab + a.tab + g * 3 * NEN;
+    double J[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double acc = 0.0;
+#pragma unroll
+        for (int n = 0; n < NEN; ++n) acc += dN[i * NEN + n] * s_x[el][j][n];
+        J[i][j] = acc;
+      }
+    double Ji[3][3];
+    const double det = det_inv<3>(J, Ji);
+    if (!(det > 0.0)) *a.bad = 1;
+#pragma unroll
+    for (int n = 0; n < NEN; ++n)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) acc += Ji[i][k] * dN[k * NEN + n];
+        s_dndx[el][g][i][n] = acc;
+      }
+    const double w = c_tab[a.tab + NGP * 3 * NEN + g];
+    const double Ee = a.elemE ? a.elemE[s_elem[el]] : a.E;
+    s_sc[el][g] = w * det * Ee;
+  }
+  __syncthreads();
+  const int el = tid >> 6, lane = tid & 63;
+  if (base + el >= a.nce) return;  // the whole wavefront leaves; no barrier follows
+  const int li = lane & 15, lk = lane >> 4;
+  f64x4 acc[NT][NT];
+#pragma unroll
+  for (int i = 0; i < NT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+  for (int ks = 0; ks < KS; ++ks) {
+    const int k = ks * 4 + lk;
+    const int g = k / 6, s = k - 6 * (k / 6);
+    const bool kv = g < NGP;
+    const double sc = kv ? s_sc[el][g] : 0.0;
+    double av[NT], bv[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int I = t * 16 + li;
+      double x = 0.0, y = 0.0;
+      if (kv && I < NE) {
+        const int nd = I / 3, c = I - 3 * nd;
+        double ga[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) ga[d] = s_dndx[el][g][d][nd];
+        x = strain_b<NEN>(ga, s, c);
+        // (D B_g)[s][I]: isotropic D (normal c0 / c1, shear c2)
+        y = s < 3 ? (s == c ? a.c0 : a.c1) * ga[c] : a.c2 * x;
+        y *= sc;
+      }
+      av[t] = x;
+      bv[t] = y;
+    }
+#pragma unroll
+    for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < NT; ++tj)
+        acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ti], bv[tj], acc[ti][tj], 0, 0, 0);
+  }
+  const int32_t e = s_elem[el];
+#pragma unroll
+  for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < NT; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int I = ti * 16 + 4 * lk + r, J = tj * 16 + li;
+        if (I < NE && J < NE) {
+          const int an = I / 3, ci = I - 3 * an, bn = J / 3, cj = J - 3 * bn;
+          const int64_t slot = a.slot[(int64_t(e) * NEN + an) * NEN + bn];
+          const int64_t row = int64_t(s_node[el][an]) * 3 + ci;
+          a.val[row + (slot * 3 + cj) * a.n] += acc[ti][tj][r];
+        }
+      }
+}
+
 // ---------------------------------------------------------------------------
 // ELL expansion: row dim*v+c gets columns dim*w+c' for w in adj(v).
 // ---------------------------------------------------------------------------
@@ -392,6 +521,16 @@ constexpr int pm_epb() {
 
 template <int DIM, int NEN, int NGP>
 hipError_t launch_asm(hipStream_t s, const KArgs& k, int variant) {
+  if (variant == 2) {
+    if constexpr (DIM == 3) {
+      const int64_t blocks = (k.nce + 3) / 4;
+      if (blocks == 0) return hipSuccess;
+      hipLaunchKernelGGL((k_assemble_mfma<NEN, NGP>), dim3(unsigned(blocks)), dim3(256), 0, s, k);
+      return hipGetLastError();
+    } else {
+      return hipErrorInvalidValue;
+    }
+  }
   if (variant == 1) {
     constexpr int EPB = pm_epb<DIM, NEN, NGP>();
     const int64_t blocks = (k.nce + EPB - 1) / EPB;

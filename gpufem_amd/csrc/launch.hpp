@@ -31,7 +31,8 @@ struct AssemblyArgs {
   double* val;
   int64_t n;
   int* bad_jac;         // set to 1 when some detJ <= 0
-  int variant;          // 0: element-major threads, 1: pair-major threads
+  int variant;          // 0: element-major threads, 1: pair-major threads,
+                        // 2: FP64 matrix cores (3D elements)
 };
 hipError_t launch_assembly_color(hipStream_t s, const AssemblyArgs& a);
 

@@ -166,10 +166,13 @@ int gf_assemble(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, gf_timing* t);
 /* Assembly kernel of this matrix: 0 = one thread per (element, node pair)
  * with the element outermost; 1 = pair-major, consecutive threads on the same
  * node pair of consecutive elements of a color (neighbouring rows in the
- * scatter).  Both add the same terms in the same order (bitwise-identical
- * values).  force_variant 0/1 fixes it; -1 assembles twice with each and
- * keeps the faster (gf_assemble does this on its first call for meshes of
- * 65536 elements or more).  ms_per_variant (may be NULL): 2 averages. */
+ * scatter); 2 (Brick8/Brick20 only, forced only) = one wavefront per element
+ * forming Ke = sum_g B^T (D B) w detJ E as one GEMM on the FP64 matrix
+ * cores.  0 and 1 add the same terms in the same order (bitwise-identical
+ * values); 2 sums in another order.  force_variant 0/1/2 fixes it; -1
+ * assembles twice with 0 and 1 and keeps the faster (gf_assemble does this on
+ * its first call for meshes of 65536 elements or more).  ms_per_variant (may
+ * be NULL): 2 averages. */
 int gf_assemble_tune(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int32_t force_variant,
                      double* ms_per_variant, int32_t* chosen);
 

@@ -97,6 +97,23 @@ def test_pair_major_assembly_is_bitwise_element_major(gpu_ctx, t, nx, ny, nz, ng
         gpu_ctx.assemble_tune(mesh, K, 2)
 
 
+@pytest.mark.parametrize("t,nx,ny,nz,ng", [(gf.GF_BRICK8, 7, 5, 4, 2), (gf.GF_BRICK8, 4, 3, 3, 3),
+                                           (gf.GF_BRICK20, 4, 3, 3, 3),
+                                           (gf.GF_BRICK20, 3, 3, 2, 2)])
+def test_matrix_core_assembly_matches_oracle(gpu_ctx, t, nx, ny, nz, ng):
+    # FP64 MFMA Ke (another summation order): within 1e-12 of the oracle
+    g, conn, coords, mesh, K, Ko = build_both(gpu_ctx, t, nx, ny, nz, ng, distort=0.15, seed=4)
+    gpu_ctx.assemble_tune(mesh, K, 2)
+    gpu_ctx.assemble(mesh, K)
+    compare(gpu_ctx, K, Ko)
+
+
+def test_matrix_core_assembly_rejects_2d(gpu_ctx):
+    g, conn, coords, mesh, K, Ko = build_both(gpu_ctx, gf.GF_Q8, 4, 3, 0, 3)
+    with pytest.raises(gf.GpufemError):
+        gpu_ctx.assemble_tune(mesh, K, 2)
+
+
 def test_penalty_parity(gpu_ctx):
     p = P.config_q4_plate()
     mesh = gpu_ctx.upload_mesh(p.elem_type, p.ngauss, p.conn, p.coords, p.E, p.nu)
