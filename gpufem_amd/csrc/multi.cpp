@@ -68,8 +68,12 @@ int run_multi(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, const double* B
   B.ncols = ncols;
   B.use_x0 = opts.use_x0;
   B.nblocks = multi_blocks(n, S);
-  B.val = m->d_val;
-  B.col = m->d_col;
+  // the block-sliced copy when the matrix's SpMV variant uses it
+  const SpmvOperands o = spmv_operands(m, P->variant);
+  B.val = is_bsell_variant(P->variant) ? o.val : m->d_val;
+  B.col = is_bsell_variant(P->variant) ? o.col : m->d_col;
+  B.bsell_d = is_bsell_variant(P->variant) ? m->dim : 0;
+  B.bs = o.bs;
   B.rownnz = m->d_rownnz;
   B.dinv = opts.precond == GF_PRECOND_JACOBI ? m->d_dinv : nullptr;
   B.b = W->b;

@@ -152,6 +152,74 @@ __global__ __launch_bounds__(kNT) void k_b_init_finish(MultiBuffers B, int nb, i
   }
 }
 
+// Block-sliced SpMM: a group of S lanes per node row of the block-sliced
+// copy (lane k = load case k), positions in grid-stride order; f(R, k, acc)
+// receives the D rows of node row R.  Each output row adds its entries in
+// ascending column order, as the ELL loop below does.
+template <int S, int D, class F>
+__device__ inline void for_group_node_rows(const MultiBuffers& B, const double* __restrict__ xin,
+                                           F&& f) {
+  int g, k;
+  groups<S>(&g, &k);
+  const int64_t npos = B.bs.nslices * 64;
+  const int64_t stride = int64_t(gridDim.x) * Tiles<S>::G;
+  for (int64_t pos = int64_t(logical_block()) * Tiles<S>::G + g; pos < npos; pos += stride) {
+    const int32_t R = B.bs.perm[pos];
+    if (R < 0) continue;
+    const int64_t sl = pos >> 6;
+    const int lp = int(pos & 63);
+    const int32_t cnt = B.rownnz[int64_t(D) * R] / D;
+    const int64_t b0 = B.bs.sptr[sl];
+    double acc[D];
+#pragma unroll
+    for (int r = 0; r < D; ++r) acc[r] = 0.0;
+    for (int32_t j = 0; j < cnt; ++j) {
+      const int64_t b = b0 + j;
+      const int64_t c = int64_t(B.col[b * 64 + lp]) * D;
+      double xv[D];
+#pragma unroll
+      for (int e = 0; e < D; ++e) xv[e] = xin[(c + e) * S + k];
+#pragma unroll
+      for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int e = 0; e < D; ++e) acc[r] += B.val[((b * D + r) * D + e) * 64 + lp] * xv[e];
+    }
+    f(R, k, acc);
+  }
+}
+
+template <int S, int D>
+__global__ __launch_bounds__(kNT) void k_b_spmm_pq_bs(MultiBuffers B) {
+  const MultiState<S>* st = static_cast<const MultiState<S>*>(B.st);
+  if (st->done) return;
+  __shared__ double sh[kNT];
+  double pq = 0.0;
+  for_group_node_rows<S, D>(B, B.p, [&](int32_t R, int k, const double* acc) {
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      const int64_t o = (int64_t(D) * R + r) * S + k;
+      B.q[o] = acc[r];
+      pq += B.p[o] * acc[r];
+    }
+  });
+  pq = column_sum<S>(pq, sh);
+  if (threadIdx.x < S) B.part1[int64_t(threadIdx.x) * gridDim.x + blockIdx.x] = pq;
+}
+
+template <int S, int D>
+__global__ __launch_bounds__(kNT) void k_b_spmm_bs(MultiBuffers B, const double* __restrict__ xin,
+                                                   double* __restrict__ yout) {
+  for_group_node_rows<S, D>(B, xin, [&](int32_t R, int k, const double* acc) {
+#pragma unroll
+    for (int r = 0; r < D; ++r) yout[(int64_t(D) * R + r) * S + k] = acc[r];
+  });
+}
+
+template <int S>
+hipError_t launch_spmm_pq(hipStream_t s, const MultiBuffers& B);
+template <int S>
+hipError_t launch_spmm(hipStream_t s, const MultiBuffers& B, const double* xin, double* yout);
+
 template <int S>
 __global__ __launch_bounds__(kNT) void k_b_spmm_pq(MultiBuffers B) {
   const MultiState<S>* st = static_cast<const MultiState<S>*>(B.st);
@@ -364,11 +432,28 @@ inline int grid_for(int64_t work, int threads, int cap) {
 }
 
 template <int S>
+hipError_t launch_spmm_pq(hipStream_t s, const MultiBuffers& B) {
+  const dim3 g(B.nblocks), t(kNT);
+  if (B.bsell_d == 2) hipLaunchKernelGGL((k_b_spmm_pq_bs<S, 2>), g, t, 0, s, B);
+  else if (B.bsell_d == 3) hipLaunchKernelGGL((k_b_spmm_pq_bs<S, 3>), g, t, 0, s, B);
+  else hipLaunchKernelGGL(k_b_spmm_pq<S>, g, t, 0, s, B);
+  return hipGetLastError();
+}
+
+template <int S>
+hipError_t launch_spmm(hipStream_t s, const MultiBuffers& B, const double* xin, double* yout) {
+  const dim3 g(B.nblocks), t(kNT);
+  if (B.bsell_d == 2) hipLaunchKernelGGL((k_b_spmm_bs<S, 2>), g, t, 0, s, B, xin, yout);
+  else if (B.bsell_d == 3) hipLaunchKernelGGL((k_b_spmm_bs<S, 3>), g, t, 0, s, B, xin, yout);
+  else hipLaunchKernelGGL(k_b_spmm<S>, g, t, 0, s, B, xin, yout);
+  return hipGetLastError();
+}
+
+template <int S>
 hipError_t setup_s(hipStream_t s, const MultiBuffers& B, double rtol, int maxit) {
   hipError_t e;
   if (B.use_x0) {
-    hipLaunchKernelGGL(k_b_spmm<S>, dim3(B.nblocks), dim3(kNT), 0, s, B, B.x, B.q);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = launch_spmm<S>(s, B, B.x, B.q)) != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_b_init<S>, dim3(B.nblocks), dim3(kNT), 0, s, B);
   if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -381,8 +466,7 @@ template <int S>
 hipError_t iteration_s(hipStream_t s, const MultiBuffers& B) {
   hipError_t e;
   const dim3 g(B.nblocks), t(kNT);
-  hipLaunchKernelGGL(k_b_spmm_pq<S>, g, t, 0, s, B);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = launch_spmm_pq<S>(s, B)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_b_alpha<S>, dim3(1), t, 0, s, B, B.nblocks);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_b_update<S>, g, t, 0, s, B);
@@ -396,8 +480,7 @@ hipError_t iteration_s(hipStream_t s, const MultiBuffers& B) {
 template <int S>
 hipError_t residual_s(hipStream_t s, const MultiBuffers& B, double* out) {
   hipError_t e;
-  hipLaunchKernelGGL(k_b_spmm<S>, dim3(B.nblocks), dim3(kNT), 0, s, B, B.x, B.q);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = launch_spmm<S>(s, B, B.x, B.q)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_b_resid<S>, dim3(B.nblocks), dim3(kNT), 0, s, B);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_b_resid_finish<S>, dim3(1), dim3(kNT), 0, s, B, B.nblocks, out);

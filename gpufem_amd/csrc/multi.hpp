@@ -25,6 +25,9 @@ struct MultiBuffers {
   double* part1;  // [S][nblocks]
   double* part2;  // [3S][nblocks]
   void* st;       // MultiState<S>
+  // block-sliced copy (bsell_d = 2 or 3: val/col are its arrays; 0: ELL)
+  int bsell_d;
+  BsellView bs;
 };
 
 struct MultiHostState {

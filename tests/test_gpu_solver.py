@@ -166,15 +166,18 @@ def test_pcg_multi_rhs_and_warm_start(gpu_ctx):
     assert sw.iterations == 0
 
 
+@pytest.mark.parametrize("variant", [0, 8])
 @pytest.mark.parametrize("nrhs", [2, 5, 32])
-def test_batched_pcg_matches_column_solves(gpu_ctx, nrhs):
-    """All load cases advance together (one SpMM per iteration); each column
-    keeps its own scalars, so it follows its single-RHS iteration."""
+def test_batched_pcg_matches_column_solves(gpu_ctx, nrhs, variant):
+    """All load cases advance together (one SpMM per iteration, on the ELL
+    arrays or on the block-sliced copy); each column keeps its own scalars,
+    so it follows its single-RHS iteration."""
     p = P.structured(gf.GF_BRICK20, 3, 3, 2)
     mesh = gpu_ctx.upload_mesh(p.elem_type, p.ngauss, p.conn, p.coords, p.E, p.nu)
     K = gpu_ctx.symbolic(mesh)
     gpu_ctx.assemble(mesh, K)
     gpu_ctx.apply_penalty(K, p.fixed_dofs)
+    gpu_ctx.tune(K, variant)
     F = P.random_top_loads(p, nrhs, 7)
     F[nrhs // 2] = 0.0                            # an empty load case converges at once
     X, st = gpu_ctx.pcg(K, F)
