@@ -94,7 +94,7 @@ def test_pair_major_assembly_is_bitwise_element_major(gpu_ctx, t, nx, ny, nz, ng
     chosen, ms = gpu_ctx.assemble_tune(mesh, K)
     assert chosen in (0, 1) and np.all(ms > 0)
     with pytest.raises(gf.GpufemError):
-        gpu_ctx.assemble_tune(mesh, K, 2)
+        gpu_ctx.assemble_tune(mesh, K, 3)
 
 
 @pytest.mark.parametrize("t,nx,ny,nz,ng", [(gf.GF_BRICK8, 7, 5, 4, 2), (gf.GF_BRICK8, 4, 3, 3, 3),

