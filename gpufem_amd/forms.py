@@ -38,7 +38,12 @@ def _setup(ctx, p):
     return mesh, K, ta
 
 
+SCHEDULE = None  # None: the library's choice; main() can force one
+
+
 def _solve(ctx, K, d_b, d_x, nrhs=1, **kw):
+    if SCHEDULE is not None:
+        kw.setdefault("schedule", SCHEDULE)
     o = solve_opts(device_ptrs=1, rtol=RTOL, **kw)
     st = _lib.SolveStats()
     ctx._check(ctx.lib.gf_pcg(ctx.handle, K.handle, d_b.ptr, d_x.ptr, nrhs, C.byref(o),
@@ -269,3 +274,27 @@ def run(ctx, names, weights):
         out[name]["wall_s"] = time.perf_counter() - t0
         _log("%s done in %.1f s" % (name, out[name]["wall_s"]))
     return out
+
+
+def main(argv=None):
+    """python -m gpufem_amd.forms [names] [--weights W]: time the named forms
+    (comma list, default all) on GPU 0 and print one JSON line."""
+    import argparse
+    import json
+    from . import Context
+    ap = argparse.ArgumentParser()
+    ap.add_argument("names", nargs="?", default="mixed,batched,grouped,learned,combined")
+    ap.add_argument("--weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                      "weights", "brick8_cantilever_64x32x32.npz"))
+    ap.add_argument("--schedule", choices=["auto", "kernels", "graph"], default="auto",
+                    help="FP64 solve schedule (kernels: no hipGraph, e.g. under a tracer)")
+    a = ap.parse_args(argv)
+    global SCHEDULE
+    from . import GF_SCHED_GRAPH, GF_SCHED_KERNELS
+    SCHEDULE = {"auto": None, "kernels": GF_SCHED_KERNELS, "graph": GF_SCHED_GRAPH}[a.schedule]
+    ctx = Context(0)
+    print(json.dumps(run(ctx, [n for n in a.names.split(",") if n], a.weights)), flush=True)
+
+
+if __name__ == "__main__":
+    main()

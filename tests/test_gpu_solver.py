@@ -166,13 +166,15 @@ def test_pcg_multi_rhs_and_warm_start(gpu_ctx):
     assert sw.iterations == 0
 
 
-@pytest.mark.parametrize("variant", [0, 8])
-@pytest.mark.parametrize("nrhs", [2, 5, 32])
-def test_batched_pcg_matches_column_solves(gpu_ctx, nrhs, variant):
+@pytest.mark.parametrize("elem,variant", [(gf.GF_BRICK20, 0), (gf.GF_BRICK20, 8),
+                                          (gf.GF_Q8, 0), (gf.GF_Q8, 7)])
+@pytest.mark.parametrize("nrhs", [2, 5, 16, 32, 40])
+def test_batched_pcg_matches_column_solves(gpu_ctx, nrhs, elem, variant):
     """All load cases advance together (one SpMM per iteration, on the ELL
-    arrays or on the block-sliced copy); each column keeps its own scalars,
-    so it follows its single-RHS iteration."""
-    p = P.structured(gf.GF_BRICK20, 3, 3, 2)
+    arrays or on the block-sliced copy; 8, 16, 32 or 64 columns wide); each
+    column keeps its own scalars, so it follows its single-RHS iteration."""
+    p = (P.structured(gf.GF_BRICK20, 3, 3, 2) if elem == gf.GF_BRICK20
+         else P.structured(gf.GF_Q8, 9, 5))
     mesh = gpu_ctx.upload_mesh(p.elem_type, p.ngauss, p.conn, p.coords, p.E, p.nu)
     K = gpu_ctx.symbolic(mesh)
     gpu_ctx.assemble(mesh, K)
