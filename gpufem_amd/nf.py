@@ -110,12 +110,17 @@ def parse_nf(text, E=E_DEFAULT, nu=NU_DEFAULT, ngauss=None, name="nf"):
     loads = np.array([_floats(L.next(), dim + 1) for _ in range(nload)]).reshape(nload, dim + 1)
     if conn.size and (conn.min() < 1 or conn.max() > nnodes):
         raise NfError("connectivity refers to a node outside 1..%d" % nnodes)
-    # force vector: assignment per load row (src/FEM/FemData.cc:370-385)
+    # force vector: assignment per load row, nonzero components only, so a
+    # later row overwrites an earlier one's nonzero components
+    # (src/FEM/FemData.cc:370-385)
     f = np.zeros(nnodes * dim)
     for row in loads:
         node = int(row[0])
+        if not 1 <= node <= nnodes:
+            raise NfError("load on a node outside 1..%d" % nnodes)
         for c in range(dim):
-            f[dim * (node - 1) + c] = row[1 + c]
+            if row[1 + c] != 0.0:
+                f[dim * (node - 1) + c] = row[1 + c]
     if ngauss is None:
         ngauss = 3 if et in (GF_Q8, GF_BRICK20) else 2
     shape = MeshShape(et, dim, nen, nnodes, nelem)

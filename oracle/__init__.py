@@ -48,6 +48,7 @@ SIGS = {
                                C.c_int, C.c_int, _P]),
     "orc_apply_penalty": (_D, [_EP, C.c_int, _I64, _P, _D]),
     "orc_force_vector": (None, [C.c_int, _I64, _I64, _P, _P]),
+    "orc_apply_elimination": (None, [_EP, C.c_int, _I64, _P]),
     "orc_dot": (_D, [_I64, _P, _P, C.c_int]),
     "orc_cg": (C.c_int, [_EP, _P, _P, C.c_int, _D, C.c_int, _P]),
     "orc_pcg_jacobi": (C.c_int, [_EP, _P, _P, C.c_int, _D, C.c_int, C.c_int, _P, _P]),
@@ -248,6 +249,11 @@ def assemble(dim, ngauss, conn, coords, E, nu, elem_E=None, skip_zero=False, nth
 def apply_penalty(K, dim, supports, coef=1e4):
     sup = np.ascontiguousarray(supports, dtype=np.int32)
     return load().orc_apply_penalty(K.m, dim, sup.shape[0], _p(sup), coef)
+
+
+def apply_elimination(K, dim, supports):
+    sup = np.ascontiguousarray(supports, dtype=np.int32).reshape(-1, dim + 1)
+    load().orc_apply_elimination(K.m, dim, sup.shape[0], _p(sup))
 
 
 def force_vector(dim, nnodes, loads):

@@ -32,6 +32,9 @@
  *                         give 4 (2D) / 8 (3D) colors.
  *   orc_assemble          dense brute-force sum of element matrices.
  *   orc_apply_penalty     closed form on a hand-built matrix.
+ *   orc_apply_elimination the solution of the eliminated system equals the
+ *                         dense solve of the free-free block (scipy), and
+ *                         the penalty solution to ~1e-9.
  *   orc_cg/orc_pcg_jacobi the reference's CG known answers
  *                         (src/Test/AxyGPUTest.cpp:201-367,
  *                         src/Test/LAopsTest.cpp:108-138) and scipy direct
@@ -749,6 +752,30 @@ double orc_apply_penalty(orc_ell* K, int dim, int64_t nsup, const int32_t* suppo
     }
   }
   return pen;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Zero substitution: the intent of FEM::ApplyConstraint(SUB)               */
+/* (src/FEM/fem.cc:146-163): row and column of every fixed DOF are zeroed   */
+/* and its diagonal set to 1.  (The reference's SetElem(.., 0) calls are    */
+/* no-ops because InputIsOK rejects zeros, SPRmatrix.cc:97-102, so its SUB  */
+/* only sets the diagonal; this is the elimination the comment describes.)  */
+/* Stored entries stay stored (their values become 0).                      */
+/* ------------------------------------------------------------------------ */
+void orc_apply_elimination(orc_ell* K, int dim, int64_t nsup, const int32_t* supports) {
+  unsigned char* fixed = (unsigned char*)calloc((size_t)K->n, 1);
+  for (int64_t i = 0; i < nsup; ++i) {
+    const int32_t* row = supports + i * (dim + 1);
+    for (int j = 1; j < 1 + dim; ++j)
+      if (row[j] == 1) fixed[(int64_t)dim * (row[0] - 1) + (j - 1)] = 1;
+  }
+  for (int64_t r = 0; r < K->n; ++r)
+    for (int32_t s = 0; s < K->rownnz[r]; ++s) {
+      const int64_t idx = r + (int64_t)s * K->n;
+      const int64_t c = K->col[idx];
+      if (fixed[r] || fixed[c]) K->val[idx] = (r == c) ? 1.0 : 0.0;
+    }
+  free(fixed);
 }
 
 /* ------------------------------------------------------------------------ */

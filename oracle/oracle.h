@@ -129,6 +129,7 @@ double orc_apply_penalty(orc_ell* K, int dim, int64_t nsup, const int32_t* suppo
 
 /* Force vector by assignment, src/FEM/FemData.cc:370-385.  loads is
  * [nload][dim+1] with (node 1-based, Fx, Fy[, Fz]).  f is zeroed first. */
+void orc_apply_elimination(orc_ell* K, int dim, int64_t nsup, const int32_t* supports);
 void orc_force_vector(int dim, int64_t nnodes, int64_t nload, const double* loads,
                       double* f);
 

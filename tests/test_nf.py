@@ -37,9 +37,32 @@ def test_q4_plate_file_blocks():
     assert p.supports.shape[1] == 3 and p.loads.shape[1] == 3
     assert np.all(jacobian_dets(p) > 0)
     # the file's loads land in the force vector by assignment
-    for row in p.loads:
-        n = int(row[0])
-        assert np.array_equal(p.f[2 * (n - 1):2 * n], row[1:])
+    assert np.array_equal(p.f, O.force_vector(2, p.coords.shape[0], p.loads))
+
+
+def test_duplicate_load_rows_keep_nonzero_components():
+    text = """%ELEMENT.Q4
+1
+1 1 1 1 1 2 3 4
+%NODE.COORD
+4
+1 0 0
+2 1 0
+3 1 1
+4 0 1
+%NODE.SUPPORT
+1
+1 1 1
+%LOAD.CASE.NODAL.FORCES
+3
+3 1.5 0.0
+2 0.0 -2.0
+3 0.0 7.0
+%END
+"""
+    p = nf.parse_nf(text)
+    assert np.array_equal(p.f, O.force_vector(2, 4, p.loads))
+    assert list(p.f[4:6]) == [1.5, 7.0]
 
 
 def test_reference_femtest_properties_q4():

@@ -131,6 +131,34 @@ def test_penalty_closed_form():
     assert D[0, 0] == 1.0 and D[3, 3] == 4.0 and D[1, 0] == -0.5
 
 
+def test_elimination_solves_the_free_block():
+    """Zero substitution: fixed rows/columns become unit vectors, the free
+    block is untouched, and the solution with f = 0 on the fixed DOFs is the
+    dense solve of the free-free block (and the penalty solution to ~1e-9)."""
+    from gpufem_amd import problems as P
+    p = P.structured(gf.GF_Q4, 12, 6)
+    K, _ = O.assemble(2, 2, p.conn, p.coords, p.E, p.nu, skip_zero=False)
+    A = K.dense()
+    O.apply_elimination(K, 2, p.supports)
+    B = K.dense()
+    fixed = np.zeros(p.ndof, bool)
+    fixed[p.fixed_dofs] = True
+    free = ~fixed
+    assert np.array_equal(B[np.ix_(free, free)], A[np.ix_(free, free)])
+    assert np.array_equal(B[fixed][:, fixed], np.eye(int(fixed.sum())))
+    assert not B[fixed][:, free].any() and not B[free][:, fixed].any()
+    f = p.f.copy()
+    f[fixed] = 0.0
+    u = np.zeros(p.ndof)
+    u[free] = np.linalg.solve(A[np.ix_(free, free)], f[free])
+    x, it, rel, _ = O.pcg_jacobi(K, f, rtol=1e-12)
+    assert np.linalg.norm(x - u) <= 1e-9 * np.linalg.norm(u)
+    Kp, _ = O.assemble(2, 2, p.conn, p.coords, p.E, p.nu, skip_zero=False)
+    O.apply_penalty(Kp, 2, p.supports)
+    xp, _, _, _ = O.pcg_jacobi(Kp, p.f, rtol=1e-12)
+    assert np.linalg.norm(xp - u) <= 1e-7 * np.linalg.norm(u)
+
+
 def test_force_vector_assignment():
     loads = np.array([[2, 1.5, 0.0], [1, 0.0, -2.0], [2, 0.0, 7.0]])
     f = O.force_vector(2, 3, loads)
