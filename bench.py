@@ -229,7 +229,30 @@ def main():
                 solves[name] = {"error": str(e)}
                 continue
             solves[name] = {"iterations": st.iterations, "seconds": st.ms_device / 1e3,
-                            "true_relres": st.true_relres, "refinements": st.refinements}
+                            "true_relres": st.true_relres, "refinements": st.refinements,
+                            "fallbacks": st.fallbacks}
+            log("%s: %d it, %.3f s, true relres %.2e" % (name, st.iterations,
+                                                         st.ms_device / 1e3, st.true_relres))
+    if args.compare == "all":
+        # the elimination alternative to the penalty, same mesh and load
+        ctx.assemble(mesh, K)
+        ctx.apply_elimination(K, dofs)
+        ctx.zero_dofs(d_b, n, dofs)
+        for name, o in [
+                ("fp64_elimination", gf.solve_opts(device_ptrs=1, rtol=1e-8, maxit=cap,
+                                                   schedule=gf.GF_SCHED_GRAPH)),
+                ("mixed_elimination", gf.solve_opts(device_ptrs=1, rtol=1e-8, maxit=cap,
+                                                    precision=gf.GF_PREC_MIXED))]:
+            log("%s: solving ..." % name)
+            try:
+                st = solve(o)
+            except gf.GpufemError as e:
+                log("%s: %s" % (name, e))
+                solves[name] = {"error": str(e)}
+                continue
+            solves[name] = {"iterations": st.iterations, "seconds": st.ms_device / 1e3,
+                            "true_relres": st.true_relres, "refinements": st.refinements,
+                            "fallbacks": st.fallbacks}
             log("%s: %d it, %.3f s, true relres %.2e" % (name, st.iterations,
                                                          st.ms_device / 1e3, st.true_relres))
     # roofline: the solver's SpMV kernel (q = K p with the p.q partials)

@@ -215,6 +215,27 @@ class Context:
                                               coef, C.byref(pen)))
         return pen.value
 
+    def apply_elimination(self, ell, dofs):
+        """Zero substitution: rows and columns of the DOFs become unit vectors."""
+        dofs = np.ascontiguousarray(dofs, dtype=np.int32)
+        self._check(self.lib.gf_apply_elimination(self.handle, ell.handle, _ptr(dofs), dofs.size))
+
+    def zero_dofs(self, dev, n, dofs, nrhs=1):
+        """Clears entries d of nrhs device vectors of length n (DevBuf or pointer)."""
+        dofs = np.ascontiguousarray(dofs, dtype=np.int32)
+        ptr = dev.ptr if isinstance(dev, DevBuf) else dev
+        self._check(self.lib.gf_zero_dofs(self.handle, ptr, n, nrhs, _ptr(dofs), dofs.size))
+
+    def load_vector(self, dim, nnodes, loads, out=None):
+        """Force vector from load rows [node, F...] on the device; returns the
+        DevBuf (out, when given, is filled)."""
+        loads = np.ascontiguousarray(loads, dtype=np.float64).reshape(-1, dim + 1)
+        if out is None:
+            out = DevBuf(self, max(nnodes * dim, 1) * 8)
+        self._check(self.lib.gf_load_vector(self.handle, dim, nnodes, _ptr(loads), loads.shape[0],
+                                            out.ptr))
+        return out
+
     def upload_ell(self, val, col, rownnz):
         """Matrix from slot-major host arrays of shape (width, n)."""
         val = np.ascontiguousarray(val, dtype=np.float64)

@@ -89,6 +89,9 @@ SIGNATURES = {
     "gf_assemble": (C.c_int, [_P, _P, _P, C.POINTER(Timing)]),
     "gf_assemble_tune": (C.c_int, [_P, _P, _P, _I32, _P, _P]),
     "gf_apply_penalty": (C.c_int, [_P, _P, _P, _I64, _D, _P]),
+    "gf_apply_elimination": (C.c_int, [_P, _P, _P, _I64]),
+    "gf_zero_dofs": (C.c_int, [_P, _P, _I64, _I32, _P, _I64]),
+    "gf_load_vector": (C.c_int, [_P, _I32, _I64, _P, _I64, _P]),
     "gf_spmv": (C.c_int, [_P, _P, _P, _P, _I32, C.POINTER(Timing)]),
     "gf_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
     "gf_pcg": (C.c_int, [_P, _P, _P, _P, _I32, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),

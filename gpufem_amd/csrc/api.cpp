@@ -536,6 +536,101 @@ int gf_apply_penalty(gf_ctx* ctx, gf_ell* m, const int32_t* dofs, int64_t nd, do
   return GF_OK;
 }
 
+namespace {
+// device copy of a host DOF list (checked against [0, n)); *out = nullptr
+// for an empty list
+int upload_dofs(gf_ctx* ctx, const char* what, const int32_t* dofs, int64_t nd, int64_t n,
+                int32_t** out) {
+  *out = nullptr;
+  for (int64_t i = 0; i < nd; ++i)
+    if (dofs[i] < 0 || dofs[i] >= n)
+      return set_error(ctx, GF_ERR_ARG, std::string(what) + ": DOF out of range");
+  if (nd == 0) return GF_OK;
+  hipError_t e = hipMalloc(out, size_t(nd) * sizeof(int32_t));
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(*out, dofs, size_t(nd) * sizeof(int32_t), hipMemcpyHostToDevice,
+                       ctx->stream);
+  if (e != hipSuccess) {
+    if (*out) (void)hipFree(*out);
+    *out = nullptr;
+    return set_error(ctx, GF_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  }
+  return GF_OK;
+}
+}  // namespace
+
+int gf_apply_elimination(gf_ctx* ctx, gf_ell* m, const int32_t* dofs, int64_t nd) {
+  if (!ctx || !m || (nd > 0 && !dofs) || nd < 0)
+    return set_error(ctx, GF_ERR_ARG, "gf_apply_elimination: bad argument");
+  int st = select_device(ctx);
+  if (st) return st;
+  int32_t* d_dofs = nullptr;
+  if ((st = upload_dofs(ctx, "gf_apply_elimination", dofs, nd, m->n, &d_dofs))) return st;
+  uint8_t* d_fixed = nullptr;
+  hipError_t e = hipMalloc(&d_fixed, size_t(m->n > 0 ? m->n : 1));
+  if (e == hipSuccess) e = hipMemsetAsync(d_fixed, 0, size_t(m->n), ctx->stream);
+  if (e == hipSuccess)
+    e = launch_eliminate(ctx->stream, m->n, m->width, d_dofs, nd, d_fixed, m->d_col,
+                         m->d_rownnz, m->d_val);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (d_fixed) (void)hipFree(d_fixed);
+  if (d_dofs) (void)hipFree(d_dofs);
+  if (e != hipSuccess)
+    return set_error(ctx, GF_ERR_HIP, std::string("gf_apply_elimination: ") + hipGetErrorString(e));
+  m->dinv_valid = false;
+  m->val32_valid = false;
+  ++m->val_version;
+  return GF_OK;
+}
+
+int gf_zero_dofs(gf_ctx* ctx, double* v, int64_t n, int32_t nrhs, const int32_t* dofs,
+                 int64_t nd) {
+  if (!ctx || (nd > 0 && (!dofs || !v)) || nd < 0 || n < 0 || nrhs < 0)
+    return set_error(ctx, GF_ERR_ARG, "gf_zero_dofs: bad argument");
+  int st = select_device(ctx);
+  if (st) return st;
+  int32_t* d_dofs = nullptr;
+  if ((st = upload_dofs(ctx, "gf_zero_dofs", dofs, nd, n, &d_dofs))) return st;
+  hipError_t e = launch_zero_dofs(ctx->stream, v, n, nrhs, d_dofs, nd);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (d_dofs) (void)hipFree(d_dofs);
+  if (e != hipSuccess)
+    return set_error(ctx, GF_ERR_HIP, std::string("gf_zero_dofs: ") + hipGetErrorString(e));
+  return GF_OK;
+}
+
+int gf_load_vector(gf_ctx* ctx, int32_t dim, int64_t nnodes, const double* loads, int64_t nload,
+                   double* f) {
+  if (!ctx || (dim != 2 && dim != 3) || nnodes < 0 || nload < 0 || (nload > 0 && !loads) ||
+      (nnodes > 0 && !f) || nload > INT32_MAX)
+    return set_error(ctx, GF_ERR_ARG, "gf_load_vector: bad argument");
+  for (int64_t i = 0; i < nload; ++i) {
+    const double node = loads[i * (dim + 1)];
+    if (!(node >= 1.0 && node <= double(nnodes)))
+      return set_error(ctx, GF_ERR_ARG, "gf_load_vector: load on a node outside 1..nnodes");
+  }
+  int st = select_device(ctx);
+  if (st) return st;
+  const int64_t n = nnodes * dim;
+  double* d_loads = nullptr;
+  int32_t* d_owner = nullptr;
+  hipError_t e = hipSuccess;
+  if (nload > 0) {
+    e = hipMalloc(&d_loads, size_t(nload) * size_t(dim + 1) * sizeof(double));
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d_loads, loads, size_t(nload) * size_t(dim + 1) * sizeof(double),
+                         hipMemcpyHostToDevice, ctx->stream);
+  }
+  if (e == hipSuccess && n > 0) e = hipMalloc(&d_owner, size_t(n) * sizeof(int32_t));
+  if (e == hipSuccess) e = launch_load_vector(ctx->stream, dim, n, d_loads, nload, d_owner, f);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (d_loads) (void)hipFree(d_loads);
+  if (d_owner) (void)hipFree(d_owner);
+  if (e != hipSuccess)
+    return set_error(ctx, GF_ERR_HIP, std::string("gf_load_vector: ") + hipGetErrorString(e));
+  return GF_OK;
+}
+
 int gf_spmv(gf_ctx* ctx, const gf_ell* m, const double* x, double* y, int32_t nrhs,
             gf_timing* t) {
   if (!ctx || !m || !x || !y || nrhs < 1)

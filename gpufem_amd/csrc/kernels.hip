@@ -408,7 +408,9 @@ __global__ __launch_bounds__(256) void k_assemble_mfma(KArgs a) {
     for (int tj = 0; tj < NT; ++tj)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int I = ti * 16 + 4 * lk + r, J = tj * 16 + li;
+        // v_mfma_f64_16x16x4f64 result: lane l, register r holds
+        // D[l/16 + 4r][l%16] (lane layout measured on gfx950)
+        const int I = ti * 16 + lk + 4 * r, J = tj * 16 + li;
         if (I < NE && J < NE) {
           const int an = I / 3, ci = I - 3 * an, bn = J / 3, cj = J - 3 * bn;
           const int64_t slot = a.slot[(int64_t(e) * NEN + an) * NEN + bn];
@@ -501,6 +503,76 @@ __global__ void k_penalty_apply(double* val, const int32_t* diag, int64_t n,
     const double p = *pen;
     for (int c = 0; c < counts[t]; ++c) x = p * x;
     *v = x;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Zero substitution (the elimination described at src/FEM/fem.cc:146-163):
+// every stored entry of a fixed row or column becomes 0, a fixed diagonal 1.
+// Slot-major sweep: consecutive threads take consecutive rows of one slot.
+// ---------------------------------------------------------------------------
+__global__ void k_mark_dofs(const int32_t* __restrict__ dofs, int64_t nd,
+                            uint8_t* __restrict__ fixed) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < nd;
+       t += int64_t(gridDim.x) * blockDim.x)
+    fixed[dofs[t]] = 1;
+}
+
+__global__ void k_eliminate(int64_t n, int32_t width, const uint8_t* __restrict__ fixed,
+                            const int32_t* __restrict__ col, const int32_t* __restrict__ rownnz,
+                            double* __restrict__ val) {
+  const int64_t total = n * int64_t(width);
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t row = t % n;
+    if (t / n >= rownnz[row]) continue;
+    const int64_t c = col[t];
+    if (fixed[row] | fixed[c]) val[t] = row == c ? 1.0 : 0.0;
+  }
+}
+
+__global__ void k_zero_dofs(double* __restrict__ v, int64_t n, int32_t nrhs,
+                            const int32_t* __restrict__ dofs, int64_t nd) {
+  const int64_t total = nd * nrhs;
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+       t += int64_t(gridDim.x) * blockDim.x)
+    v[(t / nd) * n + dofs[t % nd]] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Force vector (FemData::CalcForceVector, src/FEM/FemData.cc:370-385): rows
+// [node, F_1..F_dim] assign their nonzero components in row order.  In
+// parallel: each DOF's owner is the last row with a nonzero component for
+// it (atomic max of the row index), and only the owner writes.
+// ---------------------------------------------------------------------------
+__global__ void k_fill_f64_i32(double* __restrict__ f, int32_t* __restrict__ owner, int64_t n) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    f[t] = 0.0;
+    owner[t] = -1;
+  }
+}
+
+__global__ void k_load_owner(const double* __restrict__ loads, int64_t nload, int dim,
+                             int32_t* __restrict__ owner) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < nload * dim;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t i = t / dim;
+    const int j = int(t % dim);
+    const double* row = loads + i * (dim + 1);
+    if (row[1 + j] != 0.0) atomicMax(owner + int64_t(dim) * (int64_t(row[0]) - 1) + j, int32_t(i));
+  }
+}
+
+__global__ void k_load_assign(const double* __restrict__ loads, int64_t nload, int dim,
+                              const int32_t* __restrict__ owner, double* __restrict__ f) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < nload * dim;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t i = t / dim;
+    const int j = int(t % dim);
+    const double* row = loads + i * (dim + 1);
+    const int64_t dof = int64_t(dim) * (int64_t(row[0]) - 1) + j;
+    if (row[1 + j] != 0.0 && owner[dof] == int32_t(i)) f[dof] = row[1 + j];
   }
 }
 
@@ -621,6 +693,43 @@ hipError_t launch_penalty_factor(hipStream_t s, const double* val, const int32_t
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_max_finish, dim3(1), dim3(256), 0, s, partials, nb, coef, result);
+  return hipGetLastError();
+}
+
+hipError_t launch_eliminate(hipStream_t s, int64_t n, int32_t width, const int32_t* dofs,
+                            int64_t nd, uint8_t* fixed, const int32_t* col,
+                            const int32_t* rownnz, double* val) {
+  if (nd > 0) {
+    hipLaunchKernelGGL(k_mark_dofs, dim3(grid_for(nd, 256, 4096)), dim3(256), 0, s, dofs, nd,
+                       fixed);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  const int64_t total = n * int64_t(width);
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_eliminate, dim3(grid_for(total, 256, 65536)), dim3(256), 0, s, n, width,
+                     fixed, col, rownnz, val);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_dofs(hipStream_t s, double* v, int64_t n, int32_t nrhs,
+                            const int32_t* dofs, int64_t nd) {
+  if (nd == 0 || nrhs == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_dofs, dim3(grid_for(nd * nrhs, 256, 4096)), dim3(256), 0, s, v, n,
+                     nrhs, dofs, nd);
+  return hipGetLastError();
+}
+
+hipError_t launch_load_vector(hipStream_t s, int dim, int64_t n, const double* loads,
+                              int64_t nload, int32_t* owner, double* f) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_f64_i32, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, f, owner, n);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || nload == 0) return e;
+  const int g = grid_for(nload * dim, 256, 4096);
+  hipLaunchKernelGGL(k_load_owner, dim3(g), dim3(256), 0, s, loads, nload, dim, owner);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_load_assign, dim3(g), dim3(256), 0, s, loads, nload, dim, owner, f);
   return hipGetLastError();
 }
 

@@ -46,6 +46,20 @@ hipError_t launch_penalty_apply(hipStream_t s, double* val, const int32_t* diag,
                                 const int32_t* dofs, const int32_t* counts, int64_t nd,
                                 const double* pen);
 
+// ---- zero substitution, right-hand sides, force vector --------------------------
+// fixed[d] = 1 for the listed DOFs (fixed zeroed by the caller); then every
+// stored entry in a fixed row or column becomes 0, the diagonal 1
+hipError_t launch_eliminate(hipStream_t s, int64_t n, int32_t width, const int32_t* dofs,
+                            int64_t nd, uint8_t* fixed, const int32_t* col,
+                            const int32_t* rownnz, double* val);
+// v[k*n + d] = 0 for the listed DOFs, k < nrhs
+hipError_t launch_zero_dofs(hipStream_t s, double* v, int64_t n, int32_t nrhs,
+                            const int32_t* dofs, int64_t nd);
+// f = 0, then the nonzero components of load rows [node, F_1..F_dim] are
+// assigned, the last row naming a DOF winning (owner: n int32 scratch)
+hipError_t launch_load_vector(hipStream_t s, int dim, int64_t n, const double* loads,
+                              int64_t nload, int32_t* owner, double* f);
+
 // ---- SpMV / solver -----------------------------------------------------------------
 // SpMV variants (see solver.hip)
 // kNode2 / kNode3: thread per row over node blocks of a 2D / 3D FEM matrix,

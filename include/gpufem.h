@@ -183,6 +183,32 @@ int gf_assemble_tune(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int32_t force_
 int gf_apply_penalty(gf_ctx* ctx, gf_ell* m, const int32_t* dofs, int64_t ndofs,
                      double coef, double* pen_out);
 
+/* Zero substitution, the elimination alternative to the penalty (the intent
+ * of FEM::ApplyConstraint(SUB), src/FEM/fem.cc:146-163): for each listed
+ * DOF d (0-based), every stored entry of row d and of column d becomes 0
+ * and K_dd = 1; stored entries stay stored.  The solution then has
+ * u_d = b_d: clear the right-hand sides at those DOFs with gf_zero_dofs
+ * for fixed supports.  (The reference's own SUB reaches SetElem(.., 0),
+ * which InputIsOK turns into a no-op, SPRmatrix.cc:97-102, so it only sets
+ * K_dd = 1; this entry point does the elimination its comment describes.) */
+int gf_apply_elimination(gf_ctx* ctx, gf_ell* m, const int32_t* dofs, int64_t ndofs);
+
+/* v[k*n + d] = 0 for every listed DOF d (host list, 0-based) and
+ * right-hand side k < nrhs; v is a device pointer.  No reference
+ * counterpart (the right-hand-side half of the elimination). */
+int gf_zero_dofs(gf_ctx* ctx, double* v, int64_t n, int32_t nrhs, const int32_t* dofs,
+                 int64_t ndofs);
+
+/* Force vector on the device (FemData::CalcForceVector,
+ * src/FEM/FemData.cc:370-385): f = 0, then for each load row
+ * [node (1-based), F_1..F_dim] in order, its nonzero components are
+ * assigned (a later row overwrites an earlier row's nonzero component; a
+ * zero component overwrites nothing).  loads: host, nload x (dim+1);
+ * f: device pointer of nnodes*dim doubles.  GF_ERR_ARG for a node outside
+ * 1..nnodes. */
+int gf_load_vector(gf_ctx* ctx, int32_t dim, int64_t nnodes, const double* loads, int64_t nload,
+                   double* f);
+
 /* Y = K X for nrhs right-hand sides stored column after column
  * (x[k*n + i]).  Host pointers. */
 int gf_spmv(gf_ctx* ctx, const gf_ell* m, const double* x, double* y, int32_t nrhs,
