@@ -112,6 +112,25 @@ def probe(ctx, lib, p, K, variant, nrep=20):
                       "copy_gbs": 2 * COPY_BYTES / (ms_copy.value * 1e-3) / 1e9}), flush=True)
 
 
+def measured_traffic(workload, variant):
+    """HBM bytes per SpMV launch from the committed hardware-counter summary
+    (two rocprofv3 --pmc runs of `bench.py --probe`, FETCH_SIZE and
+    WRITE_SIZE, each scaled by what the counter reads for a copy kernel of
+    known size), when it covers this workload and SpMV variant; else None."""
+    path = os.path.join(ROOT, "profiles", "spmv_%s_hbm_bytes.md" % workload)
+    if not os.path.exists(path):
+        return None
+    kv = {}
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("traffic_") and ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+    if kv.get("traffic_workload") != workload or kv.get("traffic_spmv_variant") != str(variant):
+        return None
+    return int(kv["traffic_bytes_per_launch"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +171,18 @@ def main():
     K = ctx.symbolic(mesh)
     t_sym = time.perf_counter() - t0
     log("symbolic: width %d, nnz %d, %d colors, %.3f s" % (K.width, K.nnz, K.ncolors, t_sym))
+    # the same structure built on the device (timed after a warm-up build)
+    try:
+        ctx.symbolic(mesh, device=True).free()
+        t0 = time.perf_counter()
+        Kd = ctx.symbolic(mesh, device=True)
+        t_sym_dev = time.perf_counter() - t0
+        sym_dev_rounds = (Kd.symbolic_timing.launches - 12) // 2
+        Kd.free()
+        log("symbolic on the device: %.3f s (%d coloring rounds)" % (t_sym_dev, sym_dev_rounds))
+    except gf.GpufemError as e:
+        log("symbolic on the device: %s" % e)
+        t_sym_dev, sym_dev_rounds = float("nan"), -1
     asm_variant, asm_variant_ms = ctx.assemble_tune(mesh, K)
     log("assembly kernels (ms): %s -> %d" % (" ".join("%.3f" % v for v in asm_variant_ms),
                                              asm_variant))
@@ -298,6 +329,9 @@ def main():
             "ms_numeric": float(np.mean(asm_ms)),
             "s_symbolic_host": t_sym,
             "dof_per_s_full": n / (t_asm + t_sym),
+            "s_symbolic_device": t_sym_dev,
+            "coloring_rounds_device": sym_dev_rounds,
+            "dof_per_s_full_device_symbolic": n / (t_asm + t_sym_dev),
             "gflops_dense_convention": asm_flops / t_asm / 1e9,
             "fp64_peak_frac": asm_flops / t_asm / 1e12 / FP64_PEAK_TFLOPS,
             "kernel_variant": asm_variant,
@@ -313,7 +347,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(p.name, variant),
             "algorithmic_bytes_per_launch": spmv_bytes,
             "ms_per_launch": ms_spmv.value,
             "copy_kernel_gbs": copy_gbs,
