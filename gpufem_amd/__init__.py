@@ -182,10 +182,12 @@ class Context:
         return Mesh(self, h, elem_type, ngauss, coords.shape[0], conn.shape[0],
                     (conn, coords, elem_E))
 
-    def symbolic(self, mesh):
+    def symbolic(self, mesh, device=False):
+        """ELL structure, coloring and slot map (host phase, or device=True)."""
         h = C.c_void_p()
         t = _lib.Timing()
-        self._check(self.lib.gf_ell_symbolic(self.handle, mesh.handle, C.byref(h), C.byref(t)))
+        fn = self.lib.gf_ell_symbolic_device if device else self.lib.gf_ell_symbolic
+        self._check(fn(self.handle, mesh.handle, C.byref(h), C.byref(t)))
         m = Ell(self, h)
         m.symbolic_timing = t
         return m
@@ -485,9 +487,11 @@ class Net:
                                                   C.byref(loss) if want_loss else None))
         return loss.value if want_loss else None
 
-    def to_dofs(self, pred_dev, in_dev, sample, mask_channel, scale, x_dev):
+    def to_dofs(self, pred_dev, in_dev, sample, mask_channel, scale, x_dev, base_channel=-1,
+                base_weight=0.0):
         self.ctx._check(self.lib.gf_nn_to_dofs(self.ctx.handle, self.handle, pred_dev, in_dev,
-                                               sample, mask_channel, float(scale), x_dev))
+                                               sample, mask_channel, base_channel,
+                                               float(base_weight), float(scale), x_dev))
 
     def free(self):
         if self.handle:

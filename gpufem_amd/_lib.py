@@ -81,6 +81,7 @@ SIGNATURES = {
     "gf_mesh_free": (C.c_int, [_P]),
     "gf_mesh_set_E": (C.c_int, [_P, _P, _P]),
     "gf_ell_symbolic": (C.c_int, [_P, _P, C.POINTER(_P), C.POINTER(Timing)]),
+    "gf_ell_symbolic_device": (C.c_int, [_P, _P, C.POINTER(_P), C.POINTER(Timing)]),
     "gf_ell_info_get": (C.c_int, [_P, C.POINTER(EllInfo)]),
     "gf_ell_colors": (C.c_int, [_P, _P, _P]),
     "gf_ell_free": (C.c_int, [_P]),
@@ -117,7 +118,7 @@ SIGNATURES = {
     "gf_nn_backward": (C.c_int, [_P, _P, _P, _P, _I32, _P]),
     "gf_nn_adam": (C.c_int, [_P, _P, C.c_float, C.c_float, C.c_float, C.c_float]),
     "gf_nn_train_step": (C.c_int, [_P, _P, _P, _P, _I32, C.c_float, _P]),
-    "gf_nn_to_dofs": (C.c_int, [_P, _P, _P, _P, _I32, _I32, _D, _P]),
+    "gf_nn_to_dofs": (C.c_int, [_P, _P, _P, _P, _I32, _I32, _I32, _D, _D, _P]),
     "gf_nn_conv3d": (C.c_int, [_P, _P, _I32, _P, _I32, _P, _P, _I32, _I32, _I32, _I32,
                                C.c_float, _P, _P, _P, _I32]),
     "gf_nn_conv3d_wgrad": (C.c_int, [_P, _P, _I32, _P, _I32, _P, _P, _I32, _I32, _P, _P]),

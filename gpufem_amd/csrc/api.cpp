@@ -302,6 +302,69 @@ int gf_ell_symbolic(gf_ctx* ctx, const gf_mesh* mesh, gf_ell** out, gf_timing* t
   return GF_OK;
 }
 
+int gf_ell_symbolic_device(gf_ctx* ctx, const gf_mesh* mesh, gf_ell** out, gf_timing* t) {
+  if (!ctx || !mesh || !out)
+    return set_error(ctx, GF_ERR_ARG, "gf_ell_symbolic_device: null argument");
+  *out = nullptr;
+  int st = select_device(ctx);
+  if (st) return st;
+  const double h0 = now_ms();
+  SymbolicDevice S;
+  std::string err;
+  hipError_t e = hipEventRecord(ctx->ev_start, ctx->stream);
+  if (e != hipSuccess)
+    return set_error(ctx, GF_ERR_HIP, std::string("gf_ell_symbolic_device: ") + hipGetErrorString(e));
+  st = build_symbolic_device(ctx->stream, mesh->d_conn, mesh->nelem, mesh->nen, mesh->nnodes, &S,
+                             &err);
+  std::unique_ptr<gf_ell> m(new gf_ell());
+  auto fail = [&](int code, const std::string& msg) {
+    dfree(S.d_adj_ptr);
+    dfree(S.d_adj);
+    dfree(S.d_slot);
+    dfree(S.d_color_elems);
+    m->d_slot = nullptr;
+    m->d_color_elems = nullptr;
+    ell_release(m.get());
+    return set_error(ctx, code, "gf_ell_symbolic_device: " + msg);
+  };
+  if (st != GF_OK) return fail(st, err);
+  m->dev = ctx->device;
+  m->dim = mesh->dim;
+  m->n = mesh->nnodes * mesh->dim;
+  m->width = S.max_adj * mesh->dim;
+  m->nnz = S.nadj * mesh->dim * mesh->dim;
+  m->ncolors = S.ncolors;
+  m->color_ptr = S.color_ptr;
+  m->h_elem_color = std::move(S.elem_color);
+  m->nelem = mesh->nelem;
+  m->nen = mesh->nen;
+  m->d_slot = S.d_slot;
+  m->d_color_elems = S.d_color_elems;
+  const size_t slots = size_t(m->n) * size_t(m->width);
+#define TRY(expr, code)                                   \
+  if ((e = (expr)) != hipSuccess) return fail(code, std::string(#expr) + ": " + hipGetErrorString(e))
+  TRY(hipMalloc(&m->d_val, slots * sizeof(double)), GF_ERR_NOMEM);
+  TRY(hipMalloc(&m->d_col, slots * sizeof(int32_t)), GF_ERR_NOMEM);
+  TRY(hipMalloc(&m->d_rownnz, size_t(m->n) * sizeof(int32_t)), GF_ERR_NOMEM);
+  TRY(hipMalloc(&m->d_diag, size_t(m->n) * sizeof(int32_t)), GF_ERR_NOMEM);
+  TRY(launch_ell_expand(ctx->stream, m->n, m->width, m->dim, S.d_adj_ptr, S.d_adj, m->d_val,
+                        m->d_col, m->d_rownnz, m->d_diag), GF_ERR_HIP);
+  TRY(hipEventRecord(ctx->ev_stop, ctx->stream), GF_ERR_HIP);
+  TRY(hipStreamSynchronize(ctx->stream), GF_ERR_HIP);
+#undef TRY
+  float f = 0.f;
+  (void)hipEventElapsedTime(&f, ctx->ev_start, ctx->ev_stop);
+  dfree(S.d_adj_ptr);
+  dfree(S.d_adj);
+  if (t) {
+    t->ms_device = f;
+    t->ms_host = now_ms() - h0;
+    t->launches = S.color_rounds * 2 + 12;
+  }
+  *out = m.release();
+  return GF_OK;
+}
+
 int gf_ell_info_get(const gf_ell* m, gf_ell_info* info) {
   if (!m || !info) return set_error((gf_ctx*)nullptr, GF_ERR_ARG, "gf_ell_info_get: null argument");
   info->n = m->n;

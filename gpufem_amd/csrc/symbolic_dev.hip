@@ -1,0 +1,357 @@
+// Device symbolic phase: the same node adjacency, first-fit element coloring
+// and element -> slot map as the host phase (symbolic.cpp), built on the GPU
+// from the connectivity already resident there.
+//
+// Adjacency: node -> incident elements by counting, a scan and an atomic
+// fill (the order inside a node's list is arbitrary and nothing depends on
+// it); then one wavefront per node gathers its candidate nodes into LDS and
+// writes the distinct ones in ascending order (rank = number of distinct
+// candidates below), which is the set the host phase sorts.
+//
+// Coloring: first-fit in element order (src/FEM/femColor.cc:132-191) in
+// rounds: an uncolored element whose lower-numbered neighbours are all
+// colored takes the smallest color none of them uses.  When every
+// lower-numbered neighbour is final, this is exactly the choice the
+// sequential sweep makes, so the coloring equals the host phase's (and the
+// reference's); each round colors at least the lowest uncolored element.
+// Colors are kept as bits of a 64-bit mask (at most 64 colors).
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "gf_internal.hpp"
+#include "launch.hpp"
+#include "symbolic.hpp"
+
+namespace gf {
+namespace {
+
+constexpr int kMaxCand = 2048;  // candidate nodes per node (incident elements x nen)
+constexpr int kRoundsPerCheck = 32;
+
+inline int grid_for(int64_t work, int threads, int cap) {
+  int64_t b = (work + threads - 1) / threads;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return int(b);
+}
+
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+       t += int64_t(gridDim.x) * blockDim.x)
+    p[t] = v;
+}
+
+// err codes: 1 node id out of range, 2 too many candidates, 3 > 64 colors
+__global__ void k_incidence_count(const int32_t* __restrict__ conn, int64_t total, int64_t nnodes,
+                                  int32_t* __restrict__ cnt, int32_t* err) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int32_t v = conn[t];
+    if (v < 1 || v > nnodes) {
+      atomicMax(err, 1);
+      continue;
+    }
+    atomicAdd(cnt + (v - 1), 1);
+  }
+}
+
+__global__ void k_incidence_fill(const int32_t* __restrict__ conn, int64_t nelem, int nen,
+                                 const int64_t* __restrict__ ptr, int32_t* __restrict__ cursor,
+                                 int32_t* __restrict__ n2e) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < nelem * nen;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t v = conn[t] - 1;
+    const int32_t pos = atomicAdd(cursor + v, 1);
+    n2e[ptr[v] + pos] = int32_t(t / nen);
+  }
+}
+
+// One wavefront (= workgroup) per node.  FILL = false: distinct count per
+// node (and the maximum); FILL = true: the distinct nodes in ascending order.
+template <bool FILL>
+__global__ __launch_bounds__(64) void k_adjacency(const int32_t* __restrict__ conn, int nen,
+                                                  const int64_t* __restrict__ n2e_ptr,
+                                                  const int32_t* __restrict__ n2e, int64_t nnodes,
+                                                  int32_t* __restrict__ cnt,
+                                                  const int64_t* __restrict__ adj_ptr,
+                                                  int32_t* __restrict__ adj, int32_t* maxcnt,
+                                                  int32_t* err) {
+  __shared__ int32_t s_c[kMaxCand];
+  __shared__ uint8_t s_first[kMaxCand];
+  const int lane = threadIdx.x;
+  for (int64_t v = blockIdx.x; v < nnodes; v += gridDim.x) {
+    const int64_t p0 = n2e_ptr[v];
+    const int m = int((n2e_ptr[v + 1] - p0) * nen);
+    if (m > kMaxCand) {
+      if (lane == 0) atomicMax(err, 2);
+      continue;  // the same branch for the whole workgroup
+    }
+    if (m == 0) {  // isolated node: the diagonal only
+      if (lane == 0) {
+        if (FILL) adj[adj_ptr[v]] = int32_t(v);
+        else {
+          cnt[v] = 1;
+          atomicMax(maxcnt, 1);
+        }
+      }
+      continue;
+    }
+    for (int i = lane; i < m; i += 64)
+      s_c[i] = conn[int64_t(n2e[p0 + i / nen]) * nen + i % nen] - 1;
+    __syncthreads();
+    int local = 0;
+    for (int i = lane; i < m; i += 64) {
+      const int32_t c = s_c[i];
+      uint8_t first = 1;
+      for (int j = 0; j < i; ++j)
+        if (s_c[j] == c) {
+          first = 0;
+          break;
+        }
+      s_first[i] = first;
+      local += first;
+    }
+    __syncthreads();
+    if (FILL) {
+      const int64_t base = adj_ptr[v];
+      for (int i = lane; i < m; i += 64) {
+        if (!s_first[i]) continue;
+        const int32_t c = s_c[i];
+        int r = 0;
+        for (int j = 0; j < m; ++j) r += (s_first[j] && s_c[j] < c) ? 1 : 0;
+        adj[base + r] = c;
+      }
+    } else {
+      for (int off = 32; off > 0; off >>= 1) local += __shfl_xor(local, off);
+      if (lane == 0) {
+        cnt[v] = local;
+        atomicMax(maxcnt, local);
+      }
+    }
+    __syncthreads();  // LDS reused by the next node
+  }
+}
+
+// slot[e][a][b] = rank of node b in the sorted adjacency of node a
+__global__ void k_slot_map(const int32_t* __restrict__ conn, int64_t nelem, int nen,
+                           const int64_t* __restrict__ adj_ptr, const int32_t* __restrict__ adj,
+                           uint8_t* __restrict__ slot) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < nelem * nen;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t e = t / nen;
+    const int64_t v = conn[t] - 1;
+    const int64_t lo = adj_ptr[v], hi = adj_ptr[v + 1];
+    for (int b = 0; b < nen; ++b) {
+      const int32_t w = conn[e * nen + b] - 1;
+      int64_t l = lo, h = hi;  // first position with adj >= w
+      while (l < h) {
+        const int64_t mid = (l + h) >> 1;
+        if (adj[mid] < w) l = mid + 1;
+        else h = mid;
+      }
+      slot[t * nen + b] = uint8_t(l - lo);
+    }
+  }
+}
+
+// One round of first-fit coloring in element order.
+__global__ void k_color_round(const int32_t* __restrict__ conn, int64_t nelem, int nen,
+                              const int64_t* __restrict__ n2e_ptr,
+                              const int32_t* __restrict__ n2e, int32_t* color, int32_t* pending,
+                              int32_t* err) {
+  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < nelem;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    if (__hip_atomic_load(color + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) continue;
+    uint64_t used = 0;
+    bool ready = true;
+    for (int a = 0; a < nen && ready; ++a) {
+      const int64_t v = conn[e * nen + a] - 1;
+      for (int64_t k = n2e_ptr[v]; k < n2e_ptr[v + 1]; ++k) {
+        const int32_t f = n2e[k];
+        if (f >= e) continue;
+        const int32_t c = __hip_atomic_load(color + f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (c == 0) {
+          ready = false;
+          break;
+        }
+        used |= 1ull << (c - 1);
+      }
+    }
+    if (!ready) {
+      atomicAdd(pending, 1);
+      continue;
+    }
+    if (used == ~0ull) {
+      atomicMax(err, 3);
+      continue;
+    }
+    const int32_t c = __ffsll(static_cast<long long>(~used));  // 1-based lowest free
+    __hip_atomic_store(color + e, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+__global__ void k_iota(int32_t* p, int64_t n) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+       t += int64_t(gridDim.x) * blockDim.x)
+    p[t] = int32_t(t);
+}
+
+// device allocations released on every exit path
+struct DevScratch {
+  std::vector<void*> ptrs;
+  template <class T>
+  hipError_t alloc(T** p, size_t count) {
+    *p = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count > 0 ? count * sizeof(T) : 1);
+    if (e == hipSuccess) ptrs.push_back(*p);
+    return e;
+  }
+  ~DevScratch() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+};
+
+}  // namespace
+
+int build_symbolic_device(hipStream_t s, const int32_t* d_conn, int64_t nelem, int nen,
+                          int64_t nnodes, SymbolicDevice* out, std::string* err) {
+  DevScratch tmp;
+  hipError_t e = hipSuccess;
+  auto hip_fail = [&](const char* what) {
+    *err = std::string(what) + ": " + hipGetErrorString(e);
+    return GF_ERR_HIP;
+  };
+#define TRY(expr)                              \
+  if ((e = (expr)) != hipSuccess) return hip_fail(#expr)
+  const int64_t total = nelem * nen;
+  int32_t *d_cnt = nullptr, *d_cursor = nullptr, *d_n2e = nullptr, *d_flags = nullptr;
+  int64_t* d_n2e_ptr = nullptr;
+  TRY(tmp.alloc(&d_flags, 4));  // [0] error code, [1] max count, [2] pending
+  TRY(tmp.alloc(&d_cnt, size_t(nnodes) + 1));
+  TRY(tmp.alloc(&d_cursor, size_t(nnodes)));
+  TRY(tmp.alloc(&d_n2e_ptr, size_t(nnodes) + 1));
+  TRY(tmp.alloc(&d_n2e, size_t(total)));
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, d_flags, int64_t(4), 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(nnodes + 1, 256, 16384)), dim3(256), 0, s, d_cnt,
+                     nnodes + 1, 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(nnodes, 256, 16384)), dim3(256), 0, s, d_cursor,
+                     nnodes, 0);
+  hipLaunchKernelGGL(k_incidence_count, dim3(grid_for(total, 256, 16384)), dim3(256), 0, s,
+                     d_conn, total, nnodes, d_cnt, d_flags);
+  TRY(hipGetLastError());
+  // exclusive sums over nnodes + 1 entries (the last count is 0 -> the total)
+  auto exclusive_sum = [&](const int32_t* in, int64_t* outp, int64_t n) -> hipError_t {
+    size_t bytes = 0;
+    hipError_t r = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, outp, int(n), s);
+    if (r != hipSuccess) return r;
+    void* ws = nullptr;
+    if ((r = tmp.alloc(reinterpret_cast<char**>(&ws), bytes)) != hipSuccess) return r;
+    return hipcub::DeviceScan::ExclusiveSum(ws, bytes, in, outp, int(n), s);
+  };
+  TRY(exclusive_sum(d_cnt, d_n2e_ptr, nnodes + 1));
+  int32_t h_flags[4] = {0, 0, 0, 0};
+  TRY(hipMemcpyAsync(h_flags, d_flags, sizeof(h_flags), hipMemcpyDeviceToHost, s));
+  TRY(hipStreamSynchronize(s));
+  if (h_flags[0] == 1) {
+    *err = "connectivity references a node outside 1.." + std::to_string(nnodes);
+    return GF_ERR_ARG;
+  }
+  hipLaunchKernelGGL(k_incidence_fill, dim3(grid_for(total, 256, 16384)), dim3(256), 0, s, d_conn,
+                     nelem, nen, d_n2e_ptr, d_cursor, d_n2e);
+  TRY(hipGetLastError());
+
+  // node adjacency
+  const int gnodes = int(std::min<int64_t>(std::max<int64_t>(nnodes, 1), 65536));
+  hipLaunchKernelGGL(k_adjacency<false>, dim3(gnodes), dim3(64), 0, s, d_conn, nen, d_n2e_ptr,
+                     d_n2e, nnodes, d_cnt, nullptr, nullptr, d_flags + 1, d_flags);
+  TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, d_cnt + nnodes, int64_t(1), 0);
+  TRY(hipMalloc(&out->d_adj_ptr, (size_t(nnodes) + 1) * sizeof(int64_t)));
+  TRY(exclusive_sum(d_cnt, out->d_adj_ptr, nnodes + 1));
+  int64_t nadj = 0;
+  TRY(hipMemcpyAsync(&nadj, out->d_adj_ptr + nnodes, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  TRY(hipMemcpyAsync(h_flags, d_flags, sizeof(h_flags), hipMemcpyDeviceToHost, s));
+  TRY(hipStreamSynchronize(s));
+  if (h_flags[0] == 2) {
+    *err = "a node has more than " + std::to_string(kMaxCand) + " candidate neighbours";
+    return GF_ERR_UNSUPPORTED;
+  }
+  out->max_adj = h_flags[1];
+  out->nadj = nadj;
+  if (out->max_adj > 255) {
+    *err = "a node has " + std::to_string(out->max_adj) +
+           " neighbours; the slot map stores ranks in 8 bits";
+    return GF_ERR_UNSUPPORTED;
+  }
+  TRY(hipMalloc(&out->d_adj, size_t(nadj) * sizeof(int32_t)));
+  hipLaunchKernelGGL(k_adjacency<true>, dim3(gnodes), dim3(64), 0, s, d_conn, nen, d_n2e_ptr,
+                     d_n2e, nnodes, nullptr, out->d_adj_ptr, out->d_adj, d_flags + 1, d_flags);
+  TRY(hipGetLastError());
+  TRY(hipMalloc(&out->d_slot, size_t(total) * size_t(nen)));
+  if (total > 0)
+    hipLaunchKernelGGL(k_slot_map, dim3(grid_for(total, 256, 16384)), dim3(256), 0, s, d_conn,
+                       nelem, nen, out->d_adj_ptr, out->d_adj, out->d_slot);
+  TRY(hipGetLastError());
+
+  // first-fit coloring in rounds
+  int32_t* d_color = nullptr;
+  TRY(tmp.alloc(&d_color, size_t(nelem)));
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(nelem, 256, 16384)), dim3(256), 0, s, d_color,
+                     nelem, 0);
+  const int gel = grid_for(nelem, 256, 16384);
+  int64_t rounds = 0;
+  for (;;) {
+    for (int r = 0; r < kRoundsPerCheck; ++r) {
+      hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, d_flags + 2, int64_t(1), 0);
+      hipLaunchKernelGGL(k_color_round, dim3(gel), dim3(256), 0, s, d_conn, nelem, nen, d_n2e_ptr,
+                         d_n2e, d_color, d_flags + 2, d_flags);
+    }
+    TRY(hipGetLastError());
+    rounds += kRoundsPerCheck;
+    TRY(hipMemcpyAsync(h_flags, d_flags, sizeof(h_flags), hipMemcpyDeviceToHost, s));
+    TRY(hipStreamSynchronize(s));
+    if (h_flags[0] == 3) {
+      *err = "first-fit coloring needs more than 64 colors";
+      return GF_ERR_UNSUPPORTED;
+    }
+    if (h_flags[2] == 0) break;
+    if (rounds > nelem + kRoundsPerCheck) {  // cannot happen: every round colors one element
+      *err = "coloring did not finish";
+      return GF_ERR_HIP;
+    }
+  }
+  out->color_rounds = rounds;
+  // elements grouped by color, ascending element id inside a color (stable sort)
+  out->elem_color.resize(size_t(nelem));
+  if (nelem > 0)
+    TRY(hipMemcpyAsync(out->elem_color.data(), d_color, size_t(nelem) * sizeof(int32_t),
+                       hipMemcpyDeviceToHost, s));
+  int32_t *d_ids = nullptr, *d_keys_out = nullptr;
+  TRY(tmp.alloc(&d_ids, size_t(nelem)));
+  TRY(tmp.alloc(&d_keys_out, size_t(nelem)));
+  TRY(hipMalloc(&out->d_color_elems, size_t(nelem > 0 ? nelem : 1) * sizeof(int32_t)));
+  if (nelem > 0) {
+    hipLaunchKernelGGL(k_iota, dim3(gel), dim3(256), 0, s, d_ids, nelem);
+    size_t bytes = 0;
+    TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, d_color, d_keys_out, d_ids,
+                                           out->d_color_elems, int(nelem), 0, 7, s));
+    char* ws = nullptr;
+    TRY(tmp.alloc(&ws, bytes));
+    TRY(hipcub::DeviceRadixSort::SortPairs(ws, bytes, d_color, d_keys_out, d_ids,
+                                           out->d_color_elems, int(nelem), 0, 7, s));
+  }
+  TRY(hipStreamSynchronize(s));
+#undef TRY
+  int ncolors = 0;
+  for (int32_t c : out->elem_color) ncolors = std::max(ncolors, int(c));
+  out->ncolors = ncolors;
+  out->color_ptr.assign(size_t(ncolors) + 1, 0);
+  for (int32_t c : out->elem_color) out->color_ptr[size_t(c)]++;
+  for (int c = 0; c < ncolors; ++c) out->color_ptr[size_t(c) + 1] += out->color_ptr[size_t(c)];
+  return GF_OK;
+}
+
+}  // namespace gf

@@ -149,6 +149,17 @@ typedef struct gf_ell_info {
 /* Symbolic structure (node adjacency x dim^2, sorted), greedy element
  * coloring and the element -> slot map, built once per mesh. */
 int gf_ell_symbolic(gf_ctx* ctx, const gf_mesh* mesh, gf_ell** out, gf_timing* t);
+/* The same matrix built on the device from the uploaded connectivity: node
+ * adjacency by a node -> element incidence and one wavefront per node,
+ * first-fit coloring in element order in rounds (each element colored once
+ * its lower-numbered neighbours are), the slot map by binary search.  The
+ * structure, the colors (identical to gf_ell_symbolic's, i.e. the
+ * reference's femColor, src/FEM/femColor.cc:79-191) and the slot map equal
+ * the host phase's; replaces the ELL growth of AddElem
+ * (src/SPRmatrix/ELLmatrix.cc:91-138) and femColor without METIS.
+ * GF_ERR_UNSUPPORTED beyond 64 colors, 255 neighbours per node or 2048
+ * (incident elements x nen) per node.  t->ms_device covers the whole phase. */
+int gf_ell_symbolic_device(gf_ctx* ctx, const gf_mesh* mesh, gf_ell** out, gf_timing* t);
 int gf_ell_info_get(const gf_ell* m, gf_ell_info* info);
 /* elem_color: [nelem], colors 1..ncolors */
 int gf_ell_colors(gf_ctx* ctx, const gf_ell* m, int32_t* elem_color);
@@ -357,10 +368,15 @@ int gf_nn_adam(gf_ctx* ctx, gf_nn* net, float lr, float beta1, float beta2, floa
 /* gf_nn_backward then Adam (0.9, 0.999, 1e-8); loss_host may be NULL */
 int gf_nn_train_step(gf_ctx* ctx, gf_nn* net, const uint16_t* in_dev, const float* target_dev,
                      int32_t batch, float lr, float* loss_host);
-/* x_dev[3 i + c] = scale * pred[sample][i][c], zero at nodes whose input
- * feature channel mask_channel exceeds 1/2; x_dev is an FP64 DOF vector */
+/* x_dev[3 i + c] = scale * (pred[sample][i][c] + base_weight *
+ * in[sample][i][base_channel + c]), zero at nodes whose input feature
+ * channel mask_channel exceeds 1/2; base_channel = -1: no base term (the
+ * network then predicts the whole field, otherwise a correction to the
+ * base estimate carried in three input channels); x_dev is an FP64 DOF
+ * vector */
 int gf_nn_to_dofs(gf_ctx* ctx, gf_nn* net, const float* pred_dev, const uint16_t* in_dev,
-                  int32_t sample, int32_t mask_channel, double scale, double* x_dev);
+                  int32_t sample, int32_t mask_channel, int32_t base_channel, double base_weight,
+                  double scale, double* x_dev);
 /* One 3x3x3 convolution layer through the same kernels (zero padding):
  *   transposed = 0: out[v] = act(bias + sum_t in[s v + t] . W[t])  W: [27][cin][cout]
  *   transposed = 1: out[u] = sum_t in[(u - t)/s] . W[t]^T           W: [27][cout][cin]
