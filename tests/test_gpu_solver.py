@@ -350,6 +350,20 @@ def test_mixed_precision_meets_fp64_tolerance(gpu_ctx, t, nx, ny, nz):
     assert np.linalg.norm(xt - xo) <= 1e-9 * np.linalg.norm(xo)
 
 
+def test_mixed_graph_replay_matches_kernel_launches(gpu_ctx):
+    """The mixed solve's captured iterations (default schedule) give the
+    bitwise result of launching them one by one; a second matrix on the same
+    plan size recaptures (different arrays) and still agrees."""
+    for t, nx, ny in [(gf.GF_Q4, 24, 12), (gf.GF_Q8, 16, 8)]:
+        p, K = assembled(gpu_ctx, t, nx, ny)
+        xg, sg = gpu_ctx.pcg(K, p.f, precision=gf.GF_PREC_MIXED)
+        xk, sk = gpu_ctx.pcg(K, p.f, precision=gf.GF_PREC_MIXED, schedule=gf.GF_SCHED_KERNELS)
+        xg2, sg2 = gpu_ctx.pcg(K, p.f, precision=gf.GF_PREC_MIXED, schedule=gf.GF_SCHED_GRAPH)
+        assert sg.iterations == sk.iterations == sg2.iterations
+        assert np.array_equal(xg, xk) and np.array_equal(xg, xg2)
+        assert sg.true_relres <= 1e-8
+
+
 def test_mixed_warm_start_and_zero_rhs(gpu_ctx):
     p, K = assembled(gpu_ctx, gf.GF_Q4, 24, 12)
     x, s = gpu_ctx.pcg(K, p.f, precision=gf.GF_PREC_MIXED)
