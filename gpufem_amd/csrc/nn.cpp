@@ -490,13 +490,17 @@ int gf_nn_train_step(gf_ctx* ctx, gf_nn* net, const uint16_t* in_dev, const floa
 }
 
 int gf_nn_to_dofs(gf_ctx* ctx, gf_nn* net, const float* pred_dev, const uint16_t* in_dev,
-                  int32_t sample, int32_t mask_channel, double scale, double* x_dev) {
+                  int32_t sample, int32_t mask_channel, int32_t base_channel, double base_weight,
+                  double scale, double* x_dev) {
   if (!ctx || !net || !pred_dev || !in_dev || !x_dev || sample < 0 || sample >= net->batch_max)
     return set_error(ctx, GF_ERR_ARG, "gf_nn_to_dofs: bad argument (sample outside 0..batch_max-1)");
+  if (mask_channel < 0 || mask_channel >= kCin || base_channel < -1 || base_channel > kCin - 3)
+    return set_error(ctx, GF_ERR_ARG, "gf_nn_to_dofs: channel outside the input channels");
   GF_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   const int64_t nodes = net->lv[0].nodes();
   GF_HIP_CHECK(ctx, launch_to_dofs(ctx->stream, pred_dev + sample * nodes * kCout,
-                                   in_dev + sample * nodes * kCin, mask_channel, nodes, scale, x_dev));
+                                   in_dev + sample * nodes * kCin, mask_channel, base_channel,
+                                   base_weight, nodes, scale, x_dev));
   return GF_OK;
 }
 

@@ -463,13 +463,17 @@ __global__ void k_adam(float* p, const float* g, float* m, float* v, int64_t n, 
   }
 }
 
-__global__ void k_to_dofs(const float* pred, const uint16_t* feat, int mask_ch, int64_t nodes,
-                          double scale, double* x) {
+__global__ void k_to_dofs(const float* pred, const uint16_t* feat, int mask_ch, int base_ch,
+                          double base_w, int64_t nodes, double scale, double* x) {
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nodes;
        i += int64_t(gridDim.x) * blockDim.x) {
     const bool fixed = bf2f(feat[i * 16 + mask_ch]) > 0.5f;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) x[i * 3 + c] = fixed ? 0.0 : scale * double(pred[i * 16 + c]);
+    for (int c = 0; c < 3; ++c) {
+      double v = double(pred[i * 16 + c]);
+      if (base_ch >= 0) v += base_w * double(bf2f(feat[i * 16 + base_ch + c]));
+      x[i * 3 + c] = fixed ? 0.0 : scale * v;
+    }
   }
 }
 
@@ -612,9 +616,10 @@ hipError_t launch_adam(hipStream_t s, float* p, const float* g, float* m, float*
 }
 
 hipError_t launch_to_dofs(hipStream_t s, const float* pred, const uint16_t* feat, int mask_ch,
-                          int64_t nodes, double scale, double* x) {
-  if (mask_ch < 0 || mask_ch >= 16) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_to_dofs, dim3(grid_for(nodes)), dim3(kNT), 0, s, pred, feat, mask_ch, nodes, scale, x);
+                          int base_ch, double base_w, int64_t nodes, double scale, double* x) {
+  if (mask_ch < 0 || mask_ch >= 16 || base_ch < -1 || base_ch > 13) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_to_dofs, dim3(grid_for(nodes)), dim3(kNT), 0, s, pred, feat, mask_ch,
+                     base_ch, base_w, nodes, scale, x);
   return hipGetLastError();
 }
 

@@ -82,10 +82,11 @@ hipError_t launch_mse(hipStream_t s, const float* pred, const float* target, int
 // Adam with bias correction folded into lr_t
 hipError_t launch_adam(hipStream_t s, float* p, const float* g, float* m, float* v, int64_t n,
                        float lr_t, float beta1, float beta2, float eps);
-// x[3i + c] = scale * pred[i][c] (c < 3), zero where feature channel
-// mask_ch of node i exceeds 1/2 (the clamped nodes); one sample
+// x[3i + c] = scale * (pred[i][c] + base_w * feat[i][base_ch + c]) (c < 3;
+// no base term when base_ch < 0), zero where feature channel mask_ch of
+// node i exceeds 1/2 (the clamped nodes); one sample
 hipError_t launch_to_dofs(hipStream_t s, const float* pred, const uint16_t* feat, int mask_ch,
-                          int64_t nodes, double scale, double* x);
+                          int base_ch, double base_w, int64_t nodes, double scale, double* x);
 // zero-fill of a device buffer (a kernel, so it may sit inside a captured graph)
 hipError_t launch_fill0(hipStream_t s, void* p, int64_t bytes);
 

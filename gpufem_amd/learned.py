@@ -14,8 +14,19 @@ Features per node (16 channels, bf16):
   1..3   nodal force / s_f  (s_f = RMS of the force on the top-face nodes)
   4      1 on the clamped face, 0 elsewhere (the prediction is zeroed there)
   5..7   x / nx, y / ny, z / nz
-  8..15  zero
-Target: u / (s_f c_u), with c_u the RMS of u / s_f over the training set.
+  8..10  beam-theory displacement estimate (x, y, z) / (s_f c_beam): the
+         member's loads reduced to shear forces and bending moments along x,
+         each cross-section's bending and shear stiffness from its moduli,
+         integrated from the clamp (Timoshenko beam, plane sections); this
+         carries the non-local clamp-to-tip dependence the convolutions'
+         receptive field cannot reach
+  11..13 1 on the faces y = 0 or ny, z = 0 or nz, x = nx
+  14..15 zero
+Target: the correction to the beam estimate, (u - beam) / (s_f c_u), with
+beam the estimate as carried (bf16) in channels 8..10 and c_u the RMS of
+(u - beam) / s_f over the training set; the output layer starts at zero, so
+an untrained network reproduces the beam estimate.  Prediction:
+x0 = s_f c_u (pred + (c_beam / c_u) feat[8..10]), zero on the clamped face.
 
 Usage (trains, evaluates, writes the weights file):
   python -m gpufem_amd.learned --train 2000 --holdout 256 --epochs 20 --out W.npz
@@ -33,6 +44,8 @@ from ._lib import GF_OK  # noqa: F401
 
 MASK_CHANNEL = 4
 NCH = 16
+FEATURES = "v3-beam-correction"  # feature set and target the weights were trained on
+BASE_CHANNEL = 8  # first of the three beam-estimate channels
 HOLDOUT_BASE = 1_000_000  # member index of the first held-out sample
 
 
@@ -65,9 +78,18 @@ class Family:
                 for dx in (0, 1):
                     cnt[dz:dz + nz, dy:dy + ny, dx:dx + nx] += 1.0
         self.node_count = cnt
+        self.base_flat = base.reshape(-1, NCH)  # the constant channels, [nodes, 16]
+        base[..., 11] = (y == 0) | (y == ny)
+        base[..., 12] = (z == 0) | (z == nz)
+        base[..., 13] = (x == nx)
         top = np.zeros((nz + 1, ny + 1, nx + 1), dtype=bool)
         top[nz] = True
         self.top = top.reshape(-1)
+        self._zc = np.arange(nz + 1) - 0.5 * nz  # node offsets from the section centre
+        self._yc = np.arange(ny + 1) - 0.5 * ny
+        t = self.template
+        w = self.beam_estimate(t.elem_E, t.f)
+        self.c_beam = float(np.sqrt(np.mean(w ** 2))) / self.force_scale(t.f)
 
     def problem(self, k):
         return problems.cantilever_problem(k, self.nx, self.ny, self.nz, template=self.template)
@@ -85,18 +107,76 @@ class Family:
         F = np.asarray(f).reshape(-1, 3)
         return float(np.sqrt(np.mean(F[self.top] ** 2)))
 
+    def beam_estimate(self, elem_E, f):
+        """Timoshenko-beam displacement [nz+1, ny+1, nx+1, 3] of the member
+        with moduli elem_E and nodal forces f, clamped at x = 0: per slab
+        (element column i) the shear force V and bending moment M of the
+        forces beyond the slab centre, curvature M / (E_b I) with E_b the
+        slab's moduli weighted by the squared distance from the section's
+        centre, shear strain V / (k G A) with the slab's mean modulus; slopes
+        and deflections summed from the clamp; u_x from the axial force and
+        from the rotated plane sections."""
+        nx, ny, nz = self.nx, self.ny, self.nz
+        F = np.asarray(f).reshape(nz + 1, ny + 1, nx + 1, 3)
+        q = F.sum(axis=(0, 1))  # nodal force resultant per x station
+        E = np.asarray(elem_E).reshape(nz, ny, nx)
+        zc = np.arange(nz) + 0.5 - 0.5 * nz
+        yc = np.arange(ny) + 0.5 - 0.5 * ny
+        Ebz = (E * (zc ** 2)[:, None, None]).sum(axis=(0, 1)) / (ny * (zc ** 2).sum())
+        Eby = (E * (yc ** 2)[None, :, None]).sum(axis=(0, 1)) / (nz * (yc ** 2).sum())
+        Em = E.mean(axis=(0, 1))
+        A, Iy, Iz = ny * nz, ny * nz ** 3 / 12.0, nz * ny ** 3 / 12.0
+        G = Em / (2.0 * (1.0 + problems.NU_DEFAULT))
+        kap = 5.0 / 6.0
+        xs = np.arange(nx + 1, dtype=float)
+        mid = np.arange(nx) + 0.5
+        beyond = xs[None, :] > mid[:, None]                       # [slab, station]
+        V = beyond.astype(float) @ q                              # [slab, 3]
+        M = (beyond * (xs[None, :] - mid[:, None])) @ q           # moment arms
+        kz, ky = M[:, 2] / (Ebz * Iy), M[:, 1] / (Eby * Iz)        # curvatures
+        gz, gy = V[:, 2] / (kap * G * A), V[:, 1] / (kap * G * A)  # shear strains
+        ex = V[:, 0] / (Em * A)                                   # axial strain
+        thz = np.concatenate([[0.0], np.cumsum(kz)])              # slopes at stations
+        thy = np.concatenate([[0.0], np.cumsum(ky)])
+        wz = np.concatenate([[0.0], np.cumsum(0.5 * (thz[:-1] + thz[1:]) + gz)])
+        wy = np.concatenate([[0.0], np.cumsum(0.5 * (thy[:-1] + thy[1:]) + gy)])
+        ua = np.concatenate([[0.0], np.cumsum(ex)])
+        out = np.zeros((nz + 1, ny + 1, nx + 1, 3))
+        out[..., 0] = (ua[None, None, :] - self._zc[:, None, None] * thz[None, None, :]
+                       - self._yc[None, :, None] * thy[None, None, :])
+        out[..., 1] = wy[None, None, :]
+        out[..., 2] = wz[None, None, :]
+        return out
+
     def features(self, p):
         """(bf16 bits [nodes * 16], s_f) of problem p."""
         x = self.base.copy()
         x[..., 0] = self.node_log_modulus(p.elem_E)
         s_f = self.force_scale(p.f)
         x[..., 1:4] = (p.f / s_f).reshape(self.nz + 1, self.ny + 1, self.nx + 1, 3)
+        x[..., 8:11] = self.beam_estimate(p.elem_E, p.f) / (s_f * self.c_beam)
         return bf16_bits(x).reshape(-1), s_f
 
-    def target(self, u, s_f, c_u):
+    def beam_dofs(self, feats, s_f):
+        """The beam estimate as the features carry it (bf16), in DOF units
+        [nodes, 3]."""
+        f = np.asarray(feats, dtype=np.uint16).reshape(self.nodes, NCH)[:, BASE_CHANNEL:BASE_CHANNEL + 3]
+        return (f.astype(np.uint32) << 16).view(np.float32).astype(np.float64) * (s_f * self.c_beam)
+
+    def target(self, u, feats, s_f, c_u):
         t = np.zeros((self.nodes, 4), dtype=np.float32)
-        t[:, :3] = np.asarray(u).reshape(-1, 3) / (s_f * c_u)
+        t[:, :3] = (np.asarray(u).reshape(-1, 3) - self.beam_dofs(feats, s_f)) / (s_f * c_u)
         return t.reshape(-1)
+
+    def base_weight(self, c_u):
+        return self.c_beam / c_u
+
+    def x0(self, pred, feats, s_f, c_u):
+        """Host version of gf_nn_to_dofs: the DOF vector from one sample's
+        prediction rows pred [nodes, >= 3] and its features."""
+        x = s_f * c_u * np.asarray(pred)[:, :3].astype(np.float64) + self.beam_dofs(feats, s_f)
+        x[self.base_flat[:, MASK_CHANNEL] > 0.5] = 0.0
+        return x.reshape(-1)
 
 
 def log_default(msg):
@@ -158,6 +238,11 @@ def train(ctx, fam, feats, targets, epochs, batch, lr0=2e-3, lr1=2e-5, seed=0x5E
         raise ValueError("fewer samples than one batch")
     if net is None:
         net = Net(ctx, *fam.grid, batch)
+        # output layer from zero: the untrained prediction is the base estimate
+        p = net.params()
+        w_off = net.layers()[-1][6]
+        p[w_off:] = 0.0
+        net.set_params(p)
     in_bytes, t_bytes = fam.nodes * NCH * 2, fam.nodes * 4 * 4
     d_in = DevBuf(ctx, nb * batch * in_bytes)
     d_t = DevBuf(ctx, nb * batch * t_bytes)
@@ -208,7 +293,8 @@ def evaluate(ctx, fam, net, c_u, ks, rtol=1e-8, log=log_default):
         d_in.upload(feats)
         t0 = time.perf_counter()
         net.forward(d_in.ptr, 1, d_pred.ptr)
-        net.to_dofs(d_pred.ptr, d_in.ptr, 0, MASK_CHANNEL, s_f * c_u, d_x.ptr)
+        net.to_dofs(d_pred.ptr, d_in.ptr, 0, MASK_CHANNEL, s_f * c_u, d_x.ptr,
+                    base_channel=BASE_CHANNEL, base_weight=fam.base_weight(c_u))
         x0 = d_x.download(np.float64)  # also synchronises
         t_inf = time.perf_counter() - t0
         st1 = _pcg_dev(ctx, K, d_b, d_x, 1, rtol)
@@ -272,6 +358,9 @@ def load_weights(ctx, path, batch_max=1):
     net = Net(ctx, nx, ny, nz, batch_max)
     net.set_params(z["params"])
     meta = json.loads(bytes(z["meta"]).decode())
+    if meta.get("features") != FEATURES:
+        raise ValueError("%s was trained on features %r, this code builds %r"
+                         % (path, meta.get("features"), FEATURES))
     return net, float(z["c_u"]), meta
 
 
@@ -291,22 +380,25 @@ def main(argv=None):
     fam = Family()
     t0 = time.perf_counter()
     feats = np.empty((args.train, fam.nodes * NCH), dtype=np.uint16)
-    us, scales, its = [], [], []
+    rs, scales, its = [], [], []
     for i, (k, p, u, it) in enumerate(solve_members(ctx, fam, range(args.train), args.group)):
         feats[i], s_f = fam.features(p)
-        us.append(u.astype(np.float32))
+        rs.append(u.astype(np.float32))
         scales.append(s_f)
         its.append(it)
     t_data = time.perf_counter() - t0
-    c_u = float(np.sqrt(np.mean([np.mean((u / s) ** 2) for u, s in zip(us, scales)])))
-    targets = np.stack([fam.target(u, s, c_u) for u, s in zip(us, scales)])
-    del us
+    # output scale: RMS of the correction to the beam estimate, per unit force
+    c_u = float(np.sqrt(np.mean([np.mean(((u.reshape(-1, 3) - fam.beam_dofs(f, s)) / s) ** 2)
+                                 for u, f, s in zip(rs, feats, scales)])))
+    targets = np.stack([fam.target(u, f, s, c_u) for u, f, s in zip(rs, feats, scales)])
+    del rs
     log_default("dataset: %d members in %.1f s, c_u %.4e, mean its %.0f" % (
         args.train, t_data, c_u, np.mean(its)))
     t0 = time.perf_counter()
     net, hist = train(ctx, fam, feats, targets, args.epochs, args.batch, lr0=args.lr)
     t_train = time.perf_counter() - t0
-    meta = {"family": "brick8_cantilever_64x32x32", "train_members": [0, args.train],
+    meta = {"family": "brick8_cantilever_64x32x32", "features": FEATURES,
+            "train_members": [0, args.train],
             "epochs": args.epochs, "batch": args.batch, "loss_history": hist,
             "train_seconds": t_train, "data_seconds": t_data}
     save_weights(args.out, net, c_u, meta)
