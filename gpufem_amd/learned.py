@@ -223,12 +223,18 @@ def solve_members(ctx, fam, ks, group_size=16, rtol=1e-6, log=log_default, worke
             G.free()
 
 
-def cosine_lr(step, total, lr0, lr1):
-    return lr1 + 0.5 * (lr0 - lr1) * (1.0 + np.cos(np.pi * min(step, total) / max(total, 1)))
+def cosine_lr(step, total, lr0, lr1, warmup=0):
+    """Linear warm-up over `warmup` steps, then cosine decay from lr0 to lr1.
+    (Without normalization layers, full-size Adam steps on a fresh network
+    move whole layers coherently and blow the activations up.)"""
+    if step < warmup:
+        return lr0 * (step + 1) / warmup
+    s, T = step - warmup, max(total - warmup, 1)
+    return lr1 + 0.5 * (lr0 - lr1) * (1.0 + np.cos(np.pi * min(s, T) / T))
 
 
-def train(ctx, fam, feats, targets, epochs, batch, lr0=2e-3, lr1=2e-5, seed=0x5EED0008,
-          net=None, log=log_default):
+def train(ctx, fam, feats, targets, epochs, batch, lr0=5e-4, lr1=5e-6, seed=0x5EED0008,
+          net=None, log=log_default, warmup=1000):
     """Adam on the GPU over the dataset (feats [N, nodes*16] uint16, targets
     [N, nodes*4] float32), batch order reshuffled every epoch.  Returns the
     network and the per-epoch mean losses."""
@@ -256,7 +262,7 @@ def train(ctx, fam, feats, targets, epochs, batch, lr0=2e-3, lr1=2e-5, seed=0x5E
         t0 = time.perf_counter()
         losses = []
         for s in rng.permutation(nb):
-            lr = cosine_lr(step, total, lr0, lr1)
+            lr = cosine_lr(step, total, lr0, lr1, warmup)
             losses.append(net.train_step(d_in.at(s * batch * in_bytes), d_t.at(s * batch * t_bytes),
                                          batch, lr))
             step += 1
@@ -372,7 +378,7 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--group", type=int, default=16)
-    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr", type=float, default=5e-4)
     ap.add_argument("--out", default="gpurun_out/learned_brick8_cantilever.npz")
     ap.add_argument("--report", default="gpurun_out/learned_report.json")
     args = ap.parse_args(argv)
