@@ -50,6 +50,28 @@ hipError_t ensure_mixed_graph(gf_ctx* ctx, gf_mixed_ws* W, const MixedBuffers& B
 }
 }  // namespace
 
+int ensure_val32(gf_ctx* ctx, gf_ell* m, int variant, const float** out) {
+  if (is_bsell_variant(variant)) {
+    // FP32 copy of the block-sliced values
+    const size_t cnt = size_t(m->bslots) * 64 * m->dim * m->dim;
+    if (!m->d_bval32) GF_HIP_CHECK(ctx, hipMalloc(&m->d_bval32, cnt * sizeof(float)));
+    if (m->bval32_version != m->val_version) {
+      GF_HIP_CHECK(ctx, launch_to_f32(ctx->stream, int64_t(cnt), m->d_bval, m->d_bval32));
+      m->bval32_version = m->val_version;
+    }
+    *out = m->d_bval32;
+  } else {
+    const size_t slots = size_t(m->n) * size_t(m->width);
+    if (!m->d_val32) GF_HIP_CHECK(ctx, hipMalloc(&m->d_val32, slots * sizeof(float)));
+    if (!m->val32_valid) {
+      GF_HIP_CHECK(ctx, launch_to_f32(ctx->stream, int64_t(slots), m->d_val, m->d_val32));
+      m->val32_valid = true;
+    }
+    *out = m->d_val32;
+  }
+  return GF_OK;
+}
+
 void release_mixed(gf_solve_plan* P) {
   gf_mixed_ws* W = P->mixed;
   if (!W) return;
@@ -90,24 +112,7 @@ int run_mixed(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, PcgHostState* h
   if (st) return st;
   const SpmvOperands o = spmv_operands(m, P->variant);
   const float* val32 = nullptr;
-  if (is_bsell_variant(P->variant)) {
-    // FP32 copy of the block-sliced values
-    const size_t cnt = size_t(m->bslots) * 64 * m->dim * m->dim;
-    if (!m->d_bval32) GF_HIP_CHECK(ctx, hipMalloc(&m->d_bval32, cnt * sizeof(float)));
-    if (m->bval32_version != m->val_version) {
-      GF_HIP_CHECK(ctx, launch_to_f32(ctx->stream, int64_t(cnt), m->d_bval, m->d_bval32));
-      m->bval32_version = m->val_version;
-    }
-    val32 = m->d_bval32;
-  } else {
-    const size_t slots = size_t(n) * size_t(m->width);
-    if (!m->d_val32) GF_HIP_CHECK(ctx, hipMalloc(&m->d_val32, slots * sizeof(float)));
-    if (!m->val32_valid) {
-      GF_HIP_CHECK(ctx, launch_to_f32(ctx->stream, int64_t(slots), m->d_val, m->d_val32));
-      m->val32_valid = true;
-    }
-    val32 = m->d_val32;
-  }
+  if ((st = ensure_val32(ctx, m, P->variant, &val32))) return st;
   if (W->dinv_version != m->val_version) {
     GF_HIP_CHECK(ctx, launch_to_f32(ctx->stream, n, m->d_dinv, W->dinv32));
     W->dinv_version = m->val_version;

@@ -49,5 +49,8 @@ hipError_t read_mixed_state(hipStream_t s, const MixedBuffers& B, MixedHostState
 int run_mixed(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, PcgHostState* h,
               int* refinements);
 void release_mixed(gf_solve_plan* P);
+// the FP32 copy of the values the SpMV variant reads (ELL or block-sliced),
+// made or refreshed when stale
+int ensure_val32(gf_ctx* ctx, gf_ell* m, int variant, const float** out);
 
 }  // namespace gf

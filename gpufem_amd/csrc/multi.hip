@@ -9,10 +9,10 @@
 // of a group handling load case k of its rows; the SpMM gives each lane
 // four load cases of a row instead (fewer, wider loads per matrix entry).  A converged column keeps alpha = beta = 0 from
 // then on, freezing its solution.
-#include "device_common.hpp"
 #include "gf_internal.hpp"
 #include "launch.hpp"
 #include "multi.hpp"
+#include "multi_kernels.hpp"
 
 namespace gf {
 
@@ -32,75 +32,10 @@ struct MultiState {
 
 namespace {
 
-using namespace dev;
+using namespace mdev;
 
-template <int S>
-__device__ inline void groups(int* g, int* k) {
-  *k = threadIdx.x % S;
-  *g = threadIdx.x / S;
-}
-
-// rows of the workgroup: tiles of 256/S*8 rows would change the ownership of
-// the single-RHS kernels; here a tile is the group count times 8 rows.
-template <int S>
-struct Tiles {
-  static constexpr int G = kNT / S;   // row groups per workgroup
-  static constexpr int RT = 8 * G;    // rows per tile
-};
-
-template <int S, class F>
-__device__ inline void for_group_rows(int64_t n, F&& f) {
-  int g, k;
-  groups<S>(&g, &k);
-  const int64_t stride = int64_t(gridDim.x) * Tiles<S>::RT;
-  for (int64_t base = int64_t(logical_block()) * Tiles<S>::RT; base < n; base += stride) {
-    const int64_t end = base + Tiles<S>::RT < n ? base + Tiles<S>::RT : n;
-    for (int64_t i = base + g; i < end; i += Tiles<S>::G) f(i, k);
-  }
-}
-
-// per-column sums over the workgroup: every thread of column k gets the sum
-template <int S>
-__device__ inline double column_sum(double v, double* sh) {
-  int g, k;
-  groups<S>(&g, &k);
-  __syncthreads();
-  sh[g * S + k] = v;
-  __syncthreads();
-  double t = 0.0;
-  for (int j = 0; j < Tiles<S>::G; ++j) t += sh[j * S + k];
-  return t;
-}
-
-// Single workgroup: sums[k] = sum over b of part[(base + k) * nb + b] for the S
-// columns, in a fixed order (row group g takes b = g, g + G, ...; groups are
-// then added in order).  Result in LDS out[S], visible after the call.
-template <int S>
-__device__ inline void reduce_columns(const double* part, int nb, int base, double* sh,
-                                      double* out) {
-  int g, k;
-  groups<S>(&g, &k);
-  const double* src = part + int64_t(base + k) * nb;
-  constexpr int G = Tiles<S>::G;
-  double t = 0.0;
-  int bb = g;
-  for (; bb + 7 * G < nb; bb += 8 * G) {  // eight loads in flight, added in order
-    double v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = src[bb + u * G];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) t += v[u];
-  }
-  for (; bb < nb; bb += G) t += src[bb];
-  __syncthreads();
-  sh[g * S + k] = t;
-  __syncthreads();
-  if (g == 0) {
-    double u = 0.0;
-    for (int j = 0; j < Tiles<S>::G; ++j) u += sh[j * S + k];
-    out[k] = u;
-  }
-  __syncthreads();
+__device__ inline SpmmArgs spmm_args(const MultiBuffers& B) {
+  return SpmmArgs{B.n, B.rownnz, B.col, B.bs};
 }
 
 template <int S>
@@ -163,167 +98,6 @@ __global__ __launch_bounds__(kNT) void k_b_init_finish(MultiBuffers B, int nb, i
   }
 }
 
-// SpMM in column chunks: a row is handled by L = S / kC lanes, lane j of
-// the row owning load cases [j kC, j kC + kC), read and written as 16-byte
-// pairs.  The L lanes of a row share each matrix entry (one fetch), and
-// two or four slots are loaded before any is added, so a wave has many
-// independent loads in flight; each output entry still adds its slots in
-// ascending column order, as the single-RHS kernels do.
-constexpr int kC = 4;
-
-template <int S>
-struct Chunks {
-  static constexpr int L = S / kC;   // lanes per row
-  static constexpr int G = kNT / L;  // rows per workgroup pass
-};
-
-template <int S>
-__device__ inline void chunk_lane(int* g, int* c0) {
-  *g = threadIdx.x / Chunks<S>::L;
-  *c0 = (threadIdx.x % Chunks<S>::L) * kC;
-}
-
-using dv2 = double __attribute__((ext_vector_type(2)));
-
-__device__ inline void load_chunk(const double* __restrict__ p, double* v) {
-  const dv2 a = *reinterpret_cast<const dv2*>(p);
-  const dv2 b = *reinterpret_cast<const dv2*>(p + 2);
-  v[0] = a.x;
-  v[1] = a.y;
-  v[2] = b.x;
-  v[3] = b.y;
-}
-
-__device__ inline void store_chunk(double* p, const double* v) {
-  dv2 a, b;
-  a.x = v[0];
-  a.y = v[1];
-  b.x = v[2];
-  b.y = v[3];
-  *reinterpret_cast<dv2*>(p) = a;
-  *reinterpret_cast<dv2*>(p + 2) = b;
-}
-
-// rows of this workgroup in the tiles for_group_rows uses (the same
-// ownership as the vector kernels), G rows per pass
-template <int S, class F>
-__device__ inline void for_chunk_rows(int64_t n, F&& f) {
-  int g, c0;
-  chunk_lane<S>(&g, &c0);
-  const int64_t stride = int64_t(gridDim.x) * Tiles<S>::RT;
-  for (int64_t base = int64_t(logical_block()) * Tiles<S>::RT; base < n; base += stride) {
-    const int64_t end = base + Tiles<S>::RT < n ? base + Tiles<S>::RT : n;
-    for (int64_t i = base + g; i < end; i += Chunks<S>::G) f(i, c0);
-  }
-}
-
-// row i of the slot-major ELL matrix times columns [c0, c0 + kC) of xin
-template <int S>
-__device__ inline void ell_row_chunk(const MultiBuffers& B, const double* __restrict__ xin,
-                                     int64_t i, int c0, double* acc) {
-  const int64_t n = B.n;
-  const int32_t rn = B.rownnz[i];
-#pragma unroll
-  for (int j = 0; j < kC; ++j) acc[j] = 0.0;
-  int32_t s = 0;
-  for (; s + 4 <= rn; s += 4) {
-    double v[4], xv[4][kC];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int64_t idx = i + int64_t(s + u) * n;
-      v[u] = B.val[idx];
-      load_chunk(xin + int64_t(B.col[idx]) * S + c0, xv[u]);
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int j = 0; j < kC; ++j) acc[j] += v[u] * xv[u][j];
-  }
-  for (; s < rn; ++s) {
-    const int64_t idx = i + int64_t(s) * n;
-    double xv[kC];
-    load_chunk(xin + int64_t(B.col[idx]) * S + c0, xv);
-    const double v = B.val[idx];
-#pragma unroll
-    for (int j = 0; j < kC; ++j) acc[j] += v * xv[j];
-  }
-}
-
-// Block-sliced copy: positions of the node-row permutation in grid-stride
-// order, G per workgroup pass; f(R, c0, acc) gets the D rows of node row R.
-template <int S, int D, class F>
-__device__ inline void for_chunk_node_rows(const MultiBuffers& B, const double* __restrict__ xin,
-                                           F&& f) {
-  int g, c0;
-  chunk_lane<S>(&g, &c0);
-  const int64_t npos = B.bs.nslices * 64;
-  const int64_t stride = int64_t(gridDim.x) * Chunks<S>::G;
-  for (int64_t pos = int64_t(logical_block()) * Chunks<S>::G + g; pos < npos; pos += stride) {
-    const int32_t R = B.bs.perm[pos];
-    if (R < 0) continue;
-    const int lp = int(pos & 63);
-    const int32_t cnt = B.rownnz[int64_t(D) * R] / D;
-    const int64_t b0 = B.bs.sptr[pos >> 6];
-    double acc[D][kC];
-#pragma unroll
-    for (int r = 0; r < D; ++r)
-#pragma unroll
-      for (int j = 0; j < kC; ++j) acc[r][j] = 0.0;
-    int32_t jb = 0;
-    for (; jb + 2 <= cnt; jb += 2) {
-      double v[2][D * D], xv[2][D][kC];
-#pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const int64_t b = b0 + jb + u;
-        const int64_t c = int64_t(B.col[b * 64 + lp]) * D;
-#pragma unroll
-        for (int k = 0; k < D * D; ++k) v[u][k] = B.val[(b * (D * D) + k) * 64 + lp];
-#pragma unroll
-        for (int e = 0; e < D; ++e) load_chunk(xin + (c + e) * S + c0, xv[u][e]);
-      }
-#pragma unroll
-      for (int u = 0; u < 2; ++u)
-#pragma unroll
-        for (int r = 0; r < D; ++r)
-#pragma unroll
-          for (int e = 0; e < D; ++e)
-#pragma unroll
-            for (int j = 0; j < kC; ++j) acc[r][j] += v[u][r * D + e] * xv[u][e][j];
-    }
-    if (jb < cnt) {
-      const int64_t b = b0 + jb;
-      const int64_t c = int64_t(B.col[b * 64 + lp]) * D;
-      double v[D * D], xv[D][kC];
-#pragma unroll
-      for (int k = 0; k < D * D; ++k) v[k] = B.val[(b * (D * D) + k) * 64 + lp];
-#pragma unroll
-      for (int e = 0; e < D; ++e) load_chunk(xin + (c + e) * S + c0, xv[e]);
-#pragma unroll
-      for (int r = 0; r < D; ++r)
-#pragma unroll
-        for (int e = 0; e < D; ++e)
-#pragma unroll
-          for (int j = 0; j < kC; ++j) acc[r][j] += v[r * D + e] * xv[e][j];
-    }
-    f(R, c0, acc);
-  }
-}
-
-// per-column workgroup sums of the lanes' chunk partials into part[k][block]
-template <int S>
-__device__ inline void chunk_partials(const double* v, double* sh, double* part) {
-  int g, c0;
-  chunk_lane<S>(&g, &c0);
-#pragma unroll
-  for (int j = 0; j < kC; ++j) sh[g * S + c0 + j] = v[j];
-  __syncthreads();
-  if (threadIdx.x < S) {
-    double t = 0.0;
-    for (int q = 0; q < Chunks<S>::G; ++q) t += sh[q * S + threadIdx.x];
-    part[int64_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;
-  }
-}
-
 // q = K p and the per-column partials of p.q
 template <int S, int D>
 __global__ __launch_bounds__(kNT) void k_b_spmm_pq_bs(MultiBuffers B) {
@@ -331,7 +105,8 @@ __global__ __launch_bounds__(kNT) void k_b_spmm_pq_bs(MultiBuffers B) {
   if (st->done) return;
   __shared__ double sh[kNT * kC];
   double pq[kC] = {};
-  for_chunk_node_rows<S, D>(B, B.p, [&](int32_t R, int c0, const double (*acc)[kC]) {
+  for_chunk_node_rows<S, D>(spmm_args(B), B.val, B.p, [&](int32_t R, int c0,
+                                                          const double (*acc)[kC]) {
 #pragma unroll
     for (int r = 0; r < D; ++r) {
       const int64_t o = (int64_t(D) * R + r) * S + c0;
@@ -348,7 +123,8 @@ __global__ __launch_bounds__(kNT) void k_b_spmm_pq_bs(MultiBuffers B) {
 template <int S, int D>
 __global__ __launch_bounds__(kNT) void k_b_spmm_bs(MultiBuffers B, const double* __restrict__ xin,
                                                    double* __restrict__ yout) {
-  for_chunk_node_rows<S, D>(B, xin, [&](int32_t R, int c0, const double (*acc)[kC]) {
+  for_chunk_node_rows<S, D>(spmm_args(B), B.val, xin, [&](int32_t R, int c0,
+                                                         const double (*acc)[kC]) {
 #pragma unroll
     for (int r = 0; r < D; ++r) store_chunk(yout + (int64_t(D) * R + r) * S + c0, acc[r]);
   });
@@ -362,7 +138,7 @@ __global__ __launch_bounds__(kNT) void k_b_spmm_pq(MultiBuffers B) {
   double pq[kC] = {};
   for_chunk_rows<S>(B.n, [&](int64_t i, int c0) {
     double acc[kC], pv[kC];
-    ell_row_chunk<S>(B, B.p, i, c0, acc);
+    ell_row_chunk<S>(spmm_args(B), B.val, B.p, i, c0, acc);
     const int64_t o = i * S + c0;
     load_chunk(B.p + o, pv);
     store_chunk(B.q + o, acc);
@@ -532,7 +308,7 @@ __global__ __launch_bounds__(kNT) void k_b_spmm(MultiBuffers B, const double* __
                                                 double* __restrict__ yout) {
   for_chunk_rows<S>(B.n, [&](int64_t i, int c0) {
     double acc[kC];
-    ell_row_chunk<S>(B, xin, i, c0, acc);
+    ell_row_chunk<S>(spmm_args(B), B.val, xin, i, c0, acc);
     store_chunk(yout + i * S + c0, acc);
   });
 }

@@ -1,0 +1,274 @@
+// Device helpers shared by the batched solvers (multi.hip: FP64 block
+// PCG; multi_mixed.hip: FP32 inner iterations in FP64 refinement).
+//
+// Vectors of S load cases are stored row-major, V[i*S + k].  The vector
+// kernels give each row S lanes (lane k = load case k); the SpMM gives a
+// row L = S / kC lanes, lane j owning load cases [j kC, j kC + kC), read
+// and written as 16-byte pieces.  The L lanes of a row share each matrix
+// entry (one fetch), and two or four slots are loaded before any is added,
+// so a wave keeps many independent loads in flight; each output entry still
+// adds its slots in ascending column order, as the single-RHS kernels do.
+#pragma once
+
+#include "device_common.hpp"
+#include "launch.hpp"
+
+namespace gf {
+namespace mdev {
+
+using namespace dev;
+
+template <int S>
+__device__ inline void groups(int* g, int* k) {
+  *k = threadIdx.x % S;
+  *g = threadIdx.x / S;
+}
+
+// rows of the workgroup: tiles of 256/S*8 rows would change the ownership of
+// the single-RHS kernels; here a tile is the group count times 8 rows.
+template <int S>
+struct Tiles {
+  static constexpr int G = kNT / S;   // row groups per workgroup
+  static constexpr int RT = 8 * G;    // rows per tile
+};
+
+template <int S, class F>
+__device__ inline void for_group_rows(int64_t n, F&& f) {
+  int g, k;
+  groups<S>(&g, &k);
+  const int64_t stride = int64_t(gridDim.x) * Tiles<S>::RT;
+  for (int64_t base = int64_t(logical_block()) * Tiles<S>::RT; base < n; base += stride) {
+    const int64_t end = base + Tiles<S>::RT < n ? base + Tiles<S>::RT : n;
+    for (int64_t i = base + g; i < end; i += Tiles<S>::G) f(i, k);
+  }
+}
+
+// per-column sums over the workgroup: every thread of column k gets the sum
+template <int S>
+__device__ inline double column_sum(double v, double* sh) {
+  int g, k;
+  groups<S>(&g, &k);
+  __syncthreads();
+  sh[g * S + k] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int j = 0; j < Tiles<S>::G; ++j) t += sh[j * S + k];
+  return t;
+}
+
+// Single workgroup: sums[k] = sum over b of part[(base + k) * nb + b] for the S
+// columns, in a fixed order (row group g takes b = g, g + G, ...; groups are
+// then added in order).  Result in LDS out[S], visible after the call.
+template <int S>
+__device__ inline void reduce_columns(const double* part, int nb, int base, double* sh,
+                                      double* out) {
+  int g, k;
+  groups<S>(&g, &k);
+  const double* src = part + int64_t(base + k) * nb;
+  constexpr int G = Tiles<S>::G;
+  double t = 0.0;
+  int bb = g;
+  for (; bb + 7 * G < nb; bb += 8 * G) {  // eight loads in flight, added in order
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = src[bb + u * G];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t += v[u];
+  }
+  for (; bb < nb; bb += G) t += src[bb];
+  __syncthreads();
+  sh[g * S + k] = t;
+  __syncthreads();
+  if (g == 0) {
+    double u = 0.0;
+    for (int j = 0; j < Tiles<S>::G; ++j) u += sh[j * S + k];
+    out[k] = u;
+  }
+  __syncthreads();
+}
+
+constexpr int kC = 4;
+
+template <int S>
+struct Chunks {
+  static constexpr int L = S / kC;   // lanes per row
+  static constexpr int G = kNT / L;  // rows per workgroup pass
+};
+
+template <int S>
+__device__ inline void chunk_lane(int* g, int* c0) {
+  *g = threadIdx.x / Chunks<S>::L;
+  *c0 = (threadIdx.x % Chunks<S>::L) * kC;
+}
+
+using dv2 = double __attribute__((ext_vector_type(2)));
+using fv4 = float __attribute__((ext_vector_type(4)));
+
+__device__ inline void load_chunk(const double* __restrict__ p, double* v) {
+  const dv2 a = *reinterpret_cast<const dv2*>(p);
+  const dv2 b = *reinterpret_cast<const dv2*>(p + 2);
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
+}
+
+__device__ inline void load_chunk(const float* __restrict__ p, float* v) {
+  const fv4 a = *reinterpret_cast<const fv4*>(p);
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = a.z;
+  v[3] = a.w;
+}
+
+__device__ inline void store_chunk(double* p, const double* v) {
+  dv2 a, b;
+  a.x = v[0];
+  a.y = v[1];
+  b.x = v[2];
+  b.y = v[3];
+  *reinterpret_cast<dv2*>(p) = a;
+  *reinterpret_cast<dv2*>(p + 2) = b;
+}
+
+__device__ inline void store_chunk(float* p, const float* v) {
+  fv4 a;
+  a.x = v[0];
+  a.y = v[1];
+  a.z = v[2];
+  a.w = v[3];
+  *reinterpret_cast<fv4*>(p) = a;
+}
+
+// the arrays an SpMM reads besides the values: ELL columns and row lengths,
+// or the block-sliced copy (bs.nslices > 0 with the copy's node columns)
+struct SpmmArgs {
+  int64_t n;
+  const int32_t* rownnz;
+  const int32_t* col;
+  BsellView bs;
+};
+
+// rows of this workgroup in the tiles for_group_rows uses (the same
+// ownership as the vector kernels), G rows per pass
+template <int S, class F>
+__device__ inline void for_chunk_rows(int64_t n, F&& f) {
+  int g, c0;
+  chunk_lane<S>(&g, &c0);
+  const int64_t stride = int64_t(gridDim.x) * Tiles<S>::RT;
+  for (int64_t base = int64_t(logical_block()) * Tiles<S>::RT; base < n; base += stride) {
+    const int64_t end = base + Tiles<S>::RT < n ? base + Tiles<S>::RT : n;
+    for (int64_t i = base + g; i < end; i += Chunks<S>::G) f(i, c0);
+  }
+}
+
+// row i of the slot-major ELL matrix times load cases [c0, c0 + kC) of xin,
+// accumulated in T
+template <int S, class T, class VT>
+__device__ inline void ell_row_chunk(const SpmmArgs& A, const VT* __restrict__ val,
+                                     const T* __restrict__ xin, int64_t i, int c0, T* acc) {
+  const int64_t n = A.n;
+  const int32_t rn = A.rownnz[i];
+#pragma unroll
+  for (int j = 0; j < kC; ++j) acc[j] = T(0);
+  int32_t s = 0;
+  for (; s + 4 <= rn; s += 4) {
+    T v[4], xv[4][kC];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t idx = i + int64_t(s + u) * n;
+      v[u] = T(val[idx]);
+      load_chunk(xin + int64_t(A.col[idx]) * S + c0, xv[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int j = 0; j < kC; ++j) acc[j] += v[u] * xv[u][j];
+  }
+  for (; s < rn; ++s) {
+    const int64_t idx = i + int64_t(s) * n;
+    T xv[kC];
+    load_chunk(xin + int64_t(A.col[idx]) * S + c0, xv);
+    const T v = T(val[idx]);
+#pragma unroll
+    for (int j = 0; j < kC; ++j) acc[j] += v * xv[j];
+  }
+}
+
+// Block-sliced copy: positions of the node-row permutation in grid-stride
+// order, G per workgroup pass; f(R, c0, acc) gets the D rows of node row R.
+template <int S, int D, class T, class VT, class F>
+__device__ inline void for_chunk_node_rows(const SpmmArgs& A, const VT* __restrict__ val,
+                                           const T* __restrict__ xin, F&& f) {
+  int g, c0;
+  chunk_lane<S>(&g, &c0);
+  const int64_t npos = A.bs.nslices * 64;
+  const int64_t stride = int64_t(gridDim.x) * Chunks<S>::G;
+  for (int64_t pos = int64_t(logical_block()) * Chunks<S>::G + g; pos < npos; pos += stride) {
+    const int32_t R = A.bs.perm[pos];
+    if (R < 0) continue;
+    const int lp = int(pos & 63);
+    const int32_t cnt = A.rownnz[int64_t(D) * R] / D;
+    const int64_t b0 = A.bs.sptr[pos >> 6];
+    T acc[D][kC];
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+      for (int j = 0; j < kC; ++j) acc[r][j] = T(0);
+    int32_t jb = 0;
+    for (; jb + 2 <= cnt; jb += 2) {
+      T v[2][D * D], xv[2][D][kC];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int64_t b = b0 + jb + u;
+        const int64_t c = int64_t(A.col[b * 64 + lp]) * D;
+#pragma unroll
+        for (int k = 0; k < D * D; ++k) v[u][k] = T(val[(b * (D * D) + k) * 64 + lp]);
+#pragma unroll
+        for (int e = 0; e < D; ++e) load_chunk(xin + (c + e) * S + c0, xv[u][e]);
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+#pragma unroll
+          for (int e = 0; e < D; ++e)
+#pragma unroll
+            for (int j = 0; j < kC; ++j) acc[r][j] += v[u][r * D + e] * xv[u][e][j];
+    }
+    if (jb < cnt) {
+      const int64_t b = b0 + jb;
+      const int64_t c = int64_t(A.col[b * 64 + lp]) * D;
+      T v[D * D], xv[D][kC];
+#pragma unroll
+      for (int k = 0; k < D * D; ++k) v[k] = T(val[(b * (D * D) + k) * 64 + lp]);
+#pragma unroll
+      for (int e = 0; e < D; ++e) load_chunk(xin + (c + e) * S + c0, xv[e]);
+#pragma unroll
+      for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int e = 0; e < D; ++e)
+#pragma unroll
+          for (int j = 0; j < kC; ++j) acc[r][j] += v[r * D + e] * xv[e][j];
+    }
+    f(R, c0, acc);
+  }
+}
+
+// per-column workgroup sums of the lanes' chunk partials into part[k][block]
+template <int S>
+__device__ inline void chunk_partials(const double* v, double* sh, double* part) {
+  int g, c0;
+  chunk_lane<S>(&g, &c0);
+#pragma unroll
+  for (int j = 0; j < kC; ++j) sh[g * S + c0 + j] = v[j];
+  __syncthreads();
+  if (threadIdx.x < S) {
+    double t = 0.0;
+    for (int q = 0; q < Chunks<S>::G; ++q) t += sh[q * S + threadIdx.x];
+    part[int64_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+}  // namespace mdev
+}  // namespace gf

@@ -19,6 +19,7 @@
 #include "launch.hpp"
 #include "mixed.hpp"
 #include "multi.hpp"
+#include "multi_mixed.hpp"
 #include "solve_plan.hpp"
 
 namespace gf {
@@ -336,6 +337,7 @@ void release_solve_plan(gf_ell* m) {
   if (P->barrier) (void)hipFree(P->barrier);
   release_mixed(P);
   release_multi(P);
+  release_multi_mixed(P);
   delete P;
   m->plan = nullptr;
 }
@@ -425,11 +427,43 @@ int gf_pcg(gf_ctx* ctx, gf_ell* m, const double* b, double* x, int32_t nrhs,
     Xd = tmp.p + n * nrhs;
   }
   GF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
-  if (nrhs > 1 && opts.batched && opts.precision == GF_PREC_FP64) {
+  if (nrhs > 1 && opts.batched) {
     MultiHostState mh;
     std::vector<double> res(static_cast<size_t>(nrhs), 0.0);
-    st = run_multi(ctx, m, opts, Bd, Xd, nrhs, &mh, res.data());
-    if (st) return st;
+    int fallbacks_b = 0, refinements_b = 0;
+    bool done = false;
+    if (opts.precision == GF_PREC_MIXED) {
+      MixedMultiHostState xh;
+      st = run_multi_mixed(ctx, m, opts, Bd, Xd, nrhs, &xh, res.data());
+      if (st) return st;
+      bool broke = false;
+      for (int k = 0; k < nrhs; ++k) broke |= xh.col_status[k] == GF_ERR_BREAKDOWN;
+      if (!broke) {
+        // the same per-column record as the FP64 path; the residual is the
+        // FP64 one of the last refresh (the only one convergence is declared on)
+        mh.it = xh.it;
+        mh.done = xh.done;
+        for (int k = 0; k < nrhs; ++k) {
+          mh.col_it[k] = xh.col_it[k];
+          mh.col_status[k] = xh.col_status[k];
+          mh.rr[k] = xh.rr_true[k];
+          mh.bnorm2[k] = xh.bnorm2[k];
+          refinements_b = std::max(refinements_b, xh.col_refinements[k]);
+        }
+        done = true;
+      } else {
+        // an FP32 operator that is not positive definite: zero-start FP64
+        // batched PCG for every column, as the single-RHS mixed solve does
+        fallbacks_b = 1;
+      }
+    }
+    if (!done) {
+      gf_solve_opts o64 = opts;
+      o64.precision = GF_PREC_FP64;
+      if (fallbacks_b) o64.use_x0 = 0;
+      st = run_multi(ctx, m, o64, Bd, Xd, nrhs, &mh, res.data());
+      if (st) return st;
+    }
     GF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
     GF_HIP_CHECK(ctx, hipEventSynchronize(ctx->ev_stop));
     float msb = 0.f;
@@ -453,6 +487,8 @@ int gf_pcg(gf_ctx* ctx, gf_ell* m, const double* b, double* x, int32_t nrhs,
     }
     S.ms_device = msb;
     S.ms_host = now_ms() - h0;
+    S.refinements = refinements_b;
+    S.fallbacks = fallbacks_b;
     if (stats) *stats = S;
     if (S.status == GF_ERR_BREAKDOWN)
       return set_error(ctx, GF_ERR_BREAKDOWN, "gf_pcg: breakdown (p.Kp <= 0 or NaN)");

@@ -8,6 +8,7 @@
 
 struct gf_mixed_ws;  // mixed-precision workspace (mixed.cpp)
 struct gf_multi_ws;  // batched multi right-hand-side workspace (multi.cpp)
+struct gf_mmulti_ws;  // batched mixed-precision workspace (multi_mixed.cpp)
 
 struct gf_solve_plan {
   int64_t n = 0;
@@ -24,6 +25,7 @@ struct gf_solve_plan {
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // keyed by preconditioner use
   gf_mixed_ws* mixed = nullptr;
   gf_multi_ws* multi = nullptr;
+  gf_mmulti_ws* mmulti = nullptr;
 };
 
 namespace gf {

@@ -119,9 +119,20 @@ def form_batched(ctx, n=48, nrhs=32, sequential_sample=4):
         worst = max(worst, float(np.linalg.norm(Xb[k] - xk) / np.linalg.norm(xk)))
         _log("batched: single load case %d: %d it %.3f s" % (k, st.iterations, st.ms_device / 1e3))
     per_rhs = seq_s / sequential_sample
+    # the same block solve with FP32 inner iterations in FP64 refinement
+    stm = _solve(ctx, K, d_b, d_x, nrhs=nrhs, batched=1, precision=GF_PREC_MIXED)
+    Xm = d_x.download(np.float64).reshape(nrhs, -1)
+    dm = max(float(np.linalg.norm(Xm[k] - Xb[k]) / max(np.linalg.norm(Xb[k]), 1e-300))
+             for k in range(nrhs))
+    _log("batched: block mixed PCG %d it %.3f s, %d refinements" % (
+        stm.iterations, stm.ms_device / 1e3, stm.refinements))
     r = {"workload": p.name + "_%drhs" % nrhs, "dof": p.ndof, "nnz": K.nnz, "nrhs": nrhs,
          "assembly": asm,
          "block": _stats(stb),
+         "block_mixed": dict(_stats(stm), refinements=int(stm.refinements),
+                             fallbacks=int(stm.fallbacks),
+                             max_rel_diff_vs_block_fp64=dm,
+                             speedup_vs_sequential=per_rhs * nrhs / (stm.ms_device / 1e3)),
          "sequential": {"seconds_per_rhs": per_rhs, "iterations": seq_it,
                         "seconds_all_rhs_extrapolated": per_rhs * nrhs,
                         "sample": "first %d of %d load cases" % (sequential_sample, nrhs)},

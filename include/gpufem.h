@@ -248,8 +248,10 @@ typedef struct gf_solve_opts {
   int32_t device_ptrs; /* 1: b and x are device pointers */
   double rtol;       /* stop when ||b - K x||_2 <= rtol ||b||_2 */
   double inner_rtol; /* reserved */
-  int32_t batched;   /* nrhs > 1 (FP64): 1 = all right-hand sides advance together
-                        with one ELL SpMM per iteration; 0 = one after another */
+  int32_t batched;   /* nrhs > 1: 1 = all right-hand sides advance together with
+                        one ELL SpMM per iteration (GF_PREC_MIXED: an FP32 SpMM, each
+                        column refreshed in FP64 as the single-RHS mixed solve does);
+                        0 = one after another */
   int32_t verify;    /* 1 (default): when the recursive residual meets rtol, recompute
                         ||b - K x|| in FP64 and, while it exceeds rtol, restart PCG from
                         the current x (at most 4 times); 0: stop on the recursive

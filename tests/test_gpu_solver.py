@@ -194,6 +194,62 @@ def test_batched_pcg_matches_column_solves(gpu_ctx, nrhs, elem, variant):
     assert sw.iterations == 0
 
 
+@pytest.mark.parametrize("elem,variant", [(gf.GF_BRICK20, 0), (gf.GF_BRICK20, 8),
+                                          (gf.GF_Q8, 0), (gf.GF_Q8, 7)])
+@pytest.mark.parametrize("nrhs", [2, 5, 32, 40])
+def test_batched_mixed_meets_fp64_tolerance(gpu_ctx, nrhs, elem, variant):
+    """All load cases advance together with FP32 inner iterations (FP32 SpMM
+    over the ELL arrays or the block-sliced copy), each column refreshed in
+    FP64: every column meets the FP64 true residual, and at a tight
+    tolerance the displacements agree with the FP64 oracle to 1e-9."""
+    p = (P.structured(gf.GF_BRICK20, 3, 3, 2) if elem == gf.GF_BRICK20
+         else P.structured(gf.GF_Q8, 9, 5))
+    mesh = gpu_ctx.upload_mesh(p.elem_type, p.ngauss, p.conn, p.coords, p.E, p.nu)
+    K = gpu_ctx.symbolic(mesh)
+    gpu_ctx.assemble(mesh, K)
+    gpu_ctx.apply_penalty(K, p.fixed_dofs)
+    gpu_ctx.tune(K, variant)
+    F = P.random_top_loads(p, nrhs, 11)
+    F[nrhs // 2] = 0.0
+    X, st = gpu_ctx.pcg(K, F, precision=gf.GF_PREC_MIXED)
+    assert st.converged and st.true_relres <= 1.05e-8
+    assert st.refinements >= 1 and st.fallbacks == 0
+    assert np.all(X[nrhs // 2] == 0)
+    X64, s64 = gpu_ctx.pcg(K, F)
+    assert st.iterations <= 3 * s64.iterations + 10
+    for k in range(nrhs):   # both at 1e-8: differences at the residual level
+        assert np.linalg.norm(X[k] - X64[k]) <= 1e-5 * max(np.linalg.norm(X64[k]), 1e-300)
+    Xt, stt = gpu_ctx.pcg(K, F, precision=gf.GF_PREC_MIXED, rtol=1e-10)
+    assert stt.converged and stt.true_relres <= 1.05e-10
+    Ko, _ = O.assemble(p.dim, p.ngauss, p.conn, p.coords, p.E, p.nu, skip_zero=False)
+    O.apply_penalty(Ko, p.dim, p.supports)
+    for k in (0, nrhs - 1):
+        xo, _, _, _ = O.pcg_jacobi(Ko, F[k], rtol=1e-12)
+        assert np.linalg.norm(Xt[k] - xo) <= 1e-9 * np.linalg.norm(xo)
+    # warm start from the answer: nothing left to do
+    Xw, sw = gpu_ctx.pcg(K, F, x0=Xt, precision=gf.GF_PREC_MIXED, rtol=1e-8)
+    assert sw.iterations == 0
+    # kernel launches one by one give the graph replay's result
+    Xk, sk = gpu_ctx.pcg(K, F, precision=gf.GF_PREC_MIXED, schedule=gf.GF_SCHED_KERNELS)
+    assert sk.iterations == st.iterations and np.array_equal(Xk, X)
+
+
+def test_batched_mixed_falls_back_to_fp64(gpu_ctx):
+    """The FP32 rounding of [[1, 1-e], [1-e, 1]] (e = 1e-9) is singular: the
+    batched FP32 iteration breaks down and every column is redone by FP64
+    batched PCG from zero."""
+    e = 1e-9
+    val = np.array([[1.0, 1.0], [1 - e, 1 - e]])
+    col = np.array([[0, 1], [1, 0]], dtype=np.int32)
+    K = gpu_ctx.upload_ell(val, col, np.array([2, 2], dtype=np.int32))
+    F = np.array([[1.0, -1.0], [2.0, -2.0], [0.5, 0.25]])
+    X, st = gpu_ctx.pcg(K, F, precision=gf.GF_PREC_MIXED, rtol=1e-10)
+    assert st.converged and st.fallbacks == 1
+    A = np.array([[1.0, 1 - e], [1 - e, 1.0]])
+    for k in range(3):
+        assert np.allclose(X[k], np.linalg.solve(A, F[k]), rtol=1e-6)
+
+
 def test_pcg_error_paths(gpu_ctx):
     K = upload_oracle(gpu_ctx, diag_corner(32))
     b = np.arange(1, 33, dtype=float) ** 2
