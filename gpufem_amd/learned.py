@@ -313,6 +313,9 @@ def evaluate(ctx, fam, net, c_u, ks, rtol=1e-8, log=log_default):
             "warm_true_relres": float(st1.true_relres),
             "inference_ms": 1e3 * t_inf,
             "x0_rel_err": float(np.linalg.norm(x0 - u0) / np.linalg.norm(u0)),
+            # the untrained network's guess (the beam estimate) for comparison
+            "base_rel_err": float(np.linalg.norm(fam.x0(np.zeros((fam.nodes, 3)), feats, s_f, c_u)
+                                                 - u0) / np.linalg.norm(u0)),
             "solution_rel_diff": float(np.linalg.norm(u1 - u0) / np.linalg.norm(u0)),
         }
         out.append(rec)
@@ -344,6 +347,7 @@ def summarize(recs):
         "max_true_relres_zero": float(max(r["zero_true_relres"] for r in recs)),
         "max_true_relres_warm": float(max(r["warm_true_relres"] for r in recs)),
         "x0_rel_err_mean": float(np.mean([r["x0_rel_err"] for r in recs])),
+        "base_rel_err_mean": float(np.mean([r.get("base_rel_err", np.nan) for r in recs])),
         "max_solution_rel_diff": float(max(r["solution_rel_diff"] for r in recs)),
     }
 
