@@ -137,37 +137,6 @@ int gf_device_info(gf_ctx* ctx, int32_t* num_cu, char* name, int32_t name_len) {
 }
 
 // ---------------------------------------------------------------------------
-namespace {
-// Nodes sorted by the Morton (Z-curve) key of their coordinates, quantized
-// to 21 bits per axis over the bounding box (x in the lowest bit); ties keep
-// node order.  Node rows taken in this order are spatially compact.
-void morton_order(const double* coords, int64_t nnodes, int dim, std::vector<int32_t>* out) {
-  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-  for (int c = 0; c < dim; ++c) {
-    lo[c] = hi[c] = nnodes > 0 ? coords[c] : 0.0;
-    for (int64_t i = 1; i < nnodes; ++i) {
-      lo[c] = std::min(lo[c], coords[i * dim + c]);
-      hi[c] = std::max(hi[c], coords[i * dim + c]);
-    }
-  }
-  std::vector<std::pair<uint64_t, int32_t>> key(static_cast<size_t>(nnodes));
-  const double top = double((1u << 21) - 1);
-  for (int64_t i = 0; i < nnodes; ++i) {
-    uint64_t k = 0;
-    for (int c = 0; c < dim; ++c) {
-      const double span = hi[c] - lo[c];
-      const double t = span > 0.0 ? (coords[i * dim + c] - lo[c]) / span : 0.0;
-      const uint64_t q = uint64_t(std::min(top, std::max(0.0, t * top)));
-      for (int bit = 0; bit < 21; ++bit) k |= ((q >> bit) & 1ull) << (bit * dim + c);
-    }
-    key[size_t(i)] = {k, int32_t(i)};
-  }
-  std::sort(key.begin(), key.end());
-  out->resize(static_cast<size_t>(nnodes));
-  for (int64_t i = 0; i < nnodes; ++i) (*out)[size_t(i)] = key[size_t(i)].second;
-}
-}  // namespace
-
 int gf_mesh_upload(gf_ctx* ctx, const gf_mesh_desc* d, gf_mesh** out) {
   if (!ctx || !d || !out) return set_error(ctx, GF_ERR_ARG, "gf_mesh_upload: null argument");
   *out = nullptr;
@@ -198,7 +167,6 @@ int gf_mesh_upload(gf_ctx* ctx, const gf_mesh_desc* d, gf_mesh** out) {
   m->nu = d->nu;
   m->dev = ctx->device;
   m->h_conn.assign(d->conn, d->conn + d->nelem * ei.nen);
-  morton_order(d->coords, d->nnodes, ei.dim, &m->node_order);
   const size_t cb = size_t(d->nelem) * ei.nen * sizeof(int32_t);
   const size_t xb = size_t(d->nnodes) * ei.dim * sizeof(double);
   auto fail = [&](int code) {
@@ -284,7 +252,6 @@ int gf_ell_symbolic(gf_ctx* ctx, const gf_mesh* mesh, gf_ell** out, gf_timing* t
   m->ncolors = S.ncolors;
   m->color_ptr = S.color_ptr;
   m->h_elem_color = S.elem_color;
-  m->node_order = mesh->node_order;
   m->nelem = mesh->nelem;
   m->nen = mesh->nen;
   const size_t slots = size_t(m->n) * size_t(m->width);
@@ -369,7 +336,6 @@ int gf_ell_symbolic_device(gf_ctx* ctx, const gf_mesh* mesh, gf_ell** out, gf_ti
   m->ncolors = S.ncolors;
   m->color_ptr = S.color_ptr;
   m->h_elem_color = std::move(S.elem_color);
-  m->node_order = mesh->node_order;
   m->nelem = mesh->nelem;
   m->nen = mesh->nen;
   m->d_slot = S.d_slot;

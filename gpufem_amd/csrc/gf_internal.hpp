@@ -59,7 +59,6 @@ struct gf_mesh {
   int64_t nnodes = 0, nelem = 0;
   double E = 0, nu = 0;
   std::vector<int32_t> h_conn;  // kept for the symbolic phase
-  std::vector<int32_t> node_order;  // nodes in Morton (Z-curve) order of their coordinates
   int32_t* d_conn = nullptr;
   double* d_coords = nullptr;
   double* d_E = nullptr;  // may stay null (uniform E)
@@ -81,9 +80,6 @@ struct gf_ell {
   int32_t* d_bperm = nullptr;
   int64_t* d_bsptr = nullptr;
   int64_t bslices = 0, bslots = 0;
-  // node rows are grouped into the copy's windows in this order (the mesh's
-  // Morton order; empty: node numbering)
-  std::vector<int32_t> node_order;
   int32_t* d_bcol = nullptr;
   double* d_bval = nullptr;
   float* d_bval32 = nullptr;

@@ -81,10 +81,7 @@ hipError_t launch_build_coln(hipStream_t s, int64_t n, int32_t width, int d, con
 
 // Block-sliced ELL ("BSELL"), a solver-side copy of a d-dimensional FEM
 // matrix.  Node rows are sorted by length (descending, stable) inside windows
-// of kBsellWindow node rows -- consecutive in the mesh's Morton order when
-// the matrix comes from a mesh, so a window is a compact region whose
-// gathered columns stay few (the batched SpMM gathers S values per column),
-// else consecutive node numbers -- and dealt into slices of 64 positions (one
+// of kBsellWindow node rows and dealt into slices of 64 positions (one
 // wavefront); slice s has sptr[s+1] - sptr[s] block slots, its longest row.
 // Block slot b, lane l: node column bcol[b*64 + l]; block values
 // bval[((b*d + r)*d + e)*64 + l] = K(d R + r, d c + e) for the lane's node

@@ -133,10 +133,9 @@ int ensure_coln(gf_ctx* ctx, gf_ell* m) {
 
 BsellView bsell_view(const gf_ell* m) { return BsellView{m->d_bperm, m->d_bsptr, m->bslices}; }
 
-// Structure of the block-sliced copy: node rows (in the mesh's Morton
-// order when known) sorted by length inside windows, slices of 64
-// positions, one block slot per entry of the longest row of the slice; then
-// the node columns.
+// Structure of the block-sliced copy: node rows sorted by length inside
+// windows, slices of 64 positions, one block slot per entry of the longest
+// row of the slice; then the node columns.
 int ensure_bsell(gf_ctx* ctx, gf_ell* m) {
   if (m->d_bperm) return GF_OK;
   const int d = m->dim;
@@ -146,15 +145,12 @@ int ensure_bsell(gf_ctx* ctx, gf_ell* m) {
   std::vector<int32_t> cnt(static_cast<size_t>(N)), perm;
   for (int64_t R = 0; R < N; ++R) cnt[size_t(R)] = rn[size_t(d * R)] / d;
   const int64_t nwin = (N + kBsellWindow - 1) / kBsellWindow;
-  // windows of spatially close node rows when the mesh's order is known
-  const bool morton = int64_t(m->node_order.size()) == N;
   std::vector<int64_t> sptr(1, 0);
   std::vector<int32_t> order;
   for (int64_t w = 0; w < nwin; ++w) {
     const int64_t r0 = w * kBsellWindow, r1 = std::min<int64_t>(N, r0 + kBsellWindow);
     order.resize(size_t(r1 - r0));
-    for (int64_t R = r0; R < r1; ++R)
-      order[size_t(R - r0)] = morton ? m->node_order[size_t(R)] : int32_t(R);
+    for (int64_t R = r0; R < r1; ++R) order[size_t(R - r0)] = int32_t(R);
     std::stable_sort(order.begin(), order.end(),
                      [&](int32_t a, int32_t b) { return cnt[size_t(a)] > cnt[size_t(b)]; });
     for (size_t s0 = 0; s0 < order.size(); s0 += 64) {
