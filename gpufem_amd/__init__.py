@@ -204,7 +204,7 @@ class Context:
 
     def assemble_tune(self, mesh, ell, force_variant=-1):
         """Pick (or force) the assembly kernel; returns (chosen, ms per variant)."""
-        ms = np.zeros(2)
+        ms = np.full(_lib.GF_ASM_VARIANTS, -1.0)
         chosen = C.c_int32()
         self._check(self.lib.gf_assemble_tune(self.handle, mesh.handle, ell.handle, force_variant,
                                               _ptr(ms), C.byref(chosen)))

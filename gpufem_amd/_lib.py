@@ -24,7 +24,8 @@ GF_Q4, GF_Q8, GF_BRICK8, GF_BRICK20 = 1, 2, 3, 4
 GF_PRECOND_NONE, GF_PRECOND_JACOBI = 0, 1
 GF_PREC_FP64, GF_PREC_MIXED = 0, 1
 GF_SCHED_AUTO, GF_SCHED_KERNELS, GF_SCHED_GRAPH, GF_SCHED_PERSISTENT = 0, 1, 2, 3
-GF_SPMV_VARIANTS = 9  # entries gf_ell_tune writes into ms_per_variant
+GF_SPMV_VARIANTS = 9
+GF_ASM_VARIANTS = 4  # entries gf_assemble_tune writes into ms_per_variant
 
 
 class GridDesc(C.Structure):

@@ -228,6 +228,8 @@ static void ell_release(gf_ell* m) {
   dfree(m->d_diag);
   dfree(m->d_color_elems);
   dfree(m->d_slot);
+  dfree(m->d_n2e_ptr);
+  dfree(m->d_n2e);
   dfree(m->d_val32);
   dfree(m->d_dinv);
   release_solve_plan(m);
@@ -461,14 +463,51 @@ int gf_ell_download(gf_ctx* ctx, const gf_ell* m, double* val, int32_t* col,
 }
 
 // ---------------------------------------------------------------------------
+// node -> (element << 5 | local node) lists for the node-gather assembly,
+// ascending element id per node (its order of summation)
+static int ensure_n2e(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m) {
+  if (m->d_n2e) return GF_OK;
+  const int64_t nn = mesh->nnodes, ne = mesh->nelem;
+  const int nen = mesh->nen;
+  std::vector<int64_t> ptr(size_t(nn) + 1, 0);
+  for (int64_t i = 0; i < ne * nen; ++i) ptr[size_t(mesh->h_conn[size_t(i)])]++;
+  for (int64_t v = 0; v < nn; ++v) ptr[size_t(v) + 1] += ptr[size_t(v)];
+  std::vector<int32_t> lst(static_cast<size_t>(ptr[size_t(nn)]));
+  std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+  for (int64_t e = 0; e < ne; ++e)  // ascending e: each node's list comes out sorted
+    for (int a = 0; a < nen; ++a) {
+      const int64_t v = mesh->h_conn[size_t(e * nen + a)] - 1;
+      lst[size_t(fill[size_t(v)]++)] = int32_t((e << 5) | a);
+    }
+  GF_HIP_CHECK(ctx, hipMalloc(&m->d_n2e_ptr, ptr.size() * sizeof(int64_t)));
+  GF_HIP_CHECK(ctx, hipMalloc(&m->d_n2e, std::max<size_t>(lst.size(), 1) * sizeof(int32_t)));
+  GF_HIP_CHECK(ctx, hipMemcpy(m->d_n2e_ptr, ptr.data(), ptr.size() * sizeof(int64_t),
+                              hipMemcpyHostToDevice));
+  GF_HIP_CHECK(ctx, hipMemcpy(m->d_n2e, lst.data(), lst.size() * sizeof(int32_t),
+                              hipMemcpyHostToDevice));
+  return GF_OK;
+}
+
+// the node-gather kernel exists for the type and every node fits its slots
+static bool gather_fits(const gf_mesh* mesh, const gf_ell* m) {
+  const int cap = gather_max_slots(mesh->type);
+  return cap > 0 && m->width / m->dim <= cap && mesh->nelem < (int64_t(1) << 26);
+}
+
 static int assemble_with(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int variant, gf_timing* t) {
   const double h0 = now_ms();
+  if (variant == 3) {
+    int s3 = ensure_n2e(ctx, mesh, m);
+    if (s3) return s3;
+  }
   int* d_bad = reinterpret_cast<int*>(static_cast<char*>(ctx->d_state) + 128);
   GF_HIP_CHECK(ctx, hipMemsetAsync(d_bad, 0, sizeof(int), ctx->stream));
   int st = record_start(ctx);
   if (st) return st;
   const size_t slots = size_t(m->n) * size_t(m->width);
-  GF_HIP_CHECK(ctx, hipMemsetAsync(m->d_val, 0, slots * sizeof(double), ctx->stream));
+  // the gather writes every live slot; the scatters accumulate from zero
+  if (variant != 3)
+    GF_HIP_CHECK(ctx, hipMemsetAsync(m->d_val, 0, slots * sizeof(double), ctx->stream));
   AssemblyArgs a;
   a.type = mesh->type;
   a.ngauss = mesh->ngauss;
@@ -482,12 +521,21 @@ static int assemble_with(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int varian
   a.n = m->n;
   a.bad_jac = d_bad;
   a.variant = variant;
-  int64_t launches = 1;
-  for (int c = 0; c < m->ncolors; ++c) {
-    a.elems = m->d_color_elems + m->color_ptr[size_t(c)];
-    a.nce = m->color_ptr[size_t(c) + 1] - m->color_ptr[size_t(c)];
-    GF_HIP_CHECK(ctx, launch_assembly_color(ctx->stream, a));
+  a.n2e_ptr = m->d_n2e_ptr;
+  a.n2e = m->d_n2e;
+  a.rownnz = m->d_rownnz;
+  a.nnodes = mesh->nnodes;
+  int64_t launches = variant == 3 ? 0 : 1;
+  if (variant == 3) {
+    GF_HIP_CHECK(ctx, launch_assembly_gather(ctx->stream, a));
     ++launches;
+  } else {
+    for (int c = 0; c < m->ncolors; ++c) {
+      a.elems = m->d_color_elems + m->color_ptr[size_t(c)];
+      a.nce = m->color_ptr[size_t(c) + 1] - m->color_ptr[size_t(c)];
+      GF_HIP_CHECK(ctx, launch_assembly_color(ctx->stream, a));
+      ++launches;
+    }
   }
   double ms = 0;
   if ((st = record_stop(ctx, &ms))) return st;
@@ -518,14 +566,20 @@ int gf_assemble_tune(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int32_t force,
                      double* ms_per_variant, int32_t* chosen) {
   int st = check_assembly_args(ctx, mesh, m, "gf_assemble_tune");
   if (st) return st;
-  if (force > 2 || (force == 2 && mesh->dim != 3))
-    return set_error(ctx, GF_ERR_ARG, "gf_assemble_tune: variant must be 0, 1 or (3D) 2");
+  if (force > 3 || (force == 2 && mesh->dim != 3))
+    return set_error(ctx, GF_ERR_ARG, "gf_assemble_tune: variant must be 0, 1, 3 or (3D) 2");
+  if (force == 3 && !gather_fits(mesh, m))
+    return set_error(ctx, GF_ERR_UNSUPPORTED,
+                     "gf_assemble_tune: the node-gather kernel does not fit this mesh");
+  if (ms_per_variant)
+    for (int v = 0; v < GF_ASM_VARIANTS; ++v) ms_per_variant[v] = -1.0;
   if (force >= 0) {
     m->asm_variant = force;
   } else {
     double best = 1e300;
     int best_v = 0;
-    for (int v = 0; v < 2; ++v) {
+    for (int v : {0, 1, 3}) {
+      if (v == 3 && !gather_fits(mesh, m)) continue;
       gf_timing t1{}, t2{};
       if ((st = assemble_with(ctx, mesh, m, v, &t1)) || (st = assemble_with(ctx, mesh, m, v, &t2)))
         return st;

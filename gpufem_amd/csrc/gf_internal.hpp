@@ -90,6 +90,10 @@ struct gf_ell {
   std::vector<int64_t> color_ptr;     // [ncolors+1] into d_color_elems
   int32_t* d_color_elems = nullptr;   // element ids grouped by color
   uint8_t* d_slot = nullptr;          // [nelem][nen][nen] node rank of b in adj(a)
+  // node -> (element << 5 | local node), ascending element (node-gather
+  // assembly; lazily built)
+  int64_t* d_n2e_ptr = nullptr;
+  int32_t* d_n2e = nullptr;
   std::vector<int32_t> h_elem_color;  // [nelem], 1-based colors
   int64_t nelem = 0;
   int nen = 0;

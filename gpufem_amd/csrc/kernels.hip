@@ -421,6 +421,140 @@ __global__ __launch_bounds__(256) void k_assemble_mfma(KArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Node-gather assembly.  Thread u owns RPT rows of node v = u / (DIM / RPT)
+// (2D: both rows, 3D: one row, so consecutive threads own consecutive rows
+// and the final writes coalesce).  For every element around v (ascending
+// element id) it forms the rows of Ke that belong to v -- J, detJ, J^-1 and
+// dN/dx at each Gauss point from the element's coordinates, then the
+// node-pair blocks G_a^T D G_b w detJ E as in the scatter kernels -- and
+// adds them into per-thread LDS accumulators indexed by ELL node slot (the
+// slot map).  Each ELL entry is a sum over elements in element order; the
+// values agree with the colored scatter's to rounding.
+// ---------------------------------------------------------------------------
+struct GArgs {
+  KArgs k;
+  const int64_t* n2e_ptr;
+  const int32_t* n2e;
+  const int32_t* rownnz;
+  int64_t nnodes;
+};
+
+template <int DIM, int NEN, int NGP, int MAXS, int RPT>
+__global__ __launch_bounds__(64) void k_assemble_gather(GArgs ga) {
+  constexpr int UPN = DIM / RPT;  // threads per node
+  constexpr int ACC = MAXS * RPT * DIM;
+  __shared__ double s_acc[ACC * 64];
+  const KArgs& a = ga.k;
+  const int lane = threadIdx.x;
+  const int64_t units = ga.nnodes * UPN;
+  // grid-stride over units; every lane runs the same trip count (LDS is per lane)
+  for (int64_t u0 = int64_t(blockIdx.x) * 64; u0 < units; u0 += int64_t(gridDim.x) * 64) {
+    const int64_t u = u0 + lane;
+    const bool live = u < units;
+    const int64_t v = live ? u / UPN : 0;
+    const int r0 = live ? int(u % UPN) * RPT : 0;
+#pragma unroll 4
+    for (int q = 0; q < ACC; ++q) s_acc[q * 64 + lane] = 0.0;
+    const int64_t k0 = live ? ga.n2e_ptr[v] : 0, k1 = live ? ga.n2e_ptr[v + 1] : 0;
+    for (int64_t kk = k0; kk < k1; ++kk) {
+      const int32_t packed = ga.n2e[kk];
+      const int64_t e = packed >> 5;
+      const int la = packed & 31;
+      double x[NEN][DIM];
+#pragma unroll
+      for (int n = 0; n < NEN; ++n) {
+        const int64_t node = a.conn[e * NEN + n] - 1;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) x[n][d] = a.coords[node * DIM + d];
+      }
+      const double Ee = a.elemE ? a.elemE[e] : a.E;
+      double kl[NEN][RPT][DIM];
+#pragma unroll
+      for (int b = 0; b < NEN; ++b)
+#pragma unroll
+        for (int r = 0; r < RPT; ++r)
+#pragma unroll
+          for (int j = 0; j < DIM; ++j) kl[b][r][j] = 0.0;
+      for (int g = 0; g < NGP; ++g) {
+        const double* dN = c_tab + a.tab + g * DIM * NEN;
+        double J[DIM][DIM];
+#pragma unroll
+        for (int i = 0; i < DIM; ++i)
+#pragma unroll
+          for (int j = 0; j < DIM; ++j) {
+            double acc = 0.0;
+#pragma unroll
+            for (int n = 0; n < NEN; ++n) acc += dN[i * NEN + n] * x[n][j];
+            J[i][j] = acc;
+          }
+        double Ji[DIM][DIM];
+        const double det = det_inv<DIM>(J, Ji);
+        if (!(det > 0.0)) *a.bad = 1;
+        const double sc = c_tab[a.tab + NGP * DIM * NEN + g] * det * Ee;
+        double gA[DIM];
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+          double acc = 0.0;
+#pragma unroll
+          for (int k = 0; k < DIM; ++k) acc += Ji[i][k] * dN[k * NEN + la];
+          gA[i] = acc;
+        }
+#pragma unroll
+        for (int b = 0; b < NEN; ++b) {
+          double gB[DIM];
+#pragma unroll
+          for (int i = 0; i < DIM; ++i) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) acc += Ji[i][k] * dN[k * NEN + b];
+            gB[i] = acc;
+          }
+          double dot = 0.0;
+#pragma unroll
+          for (int i = 0; i < DIM; ++i) dot += gA[i] * gB[i];
+#pragma unroll
+          for (int r = 0; r < RPT; ++r) {
+            const int i = r0 + r;
+#pragma unroll
+            for (int j = 0; j < DIM; ++j) {
+              double kij;
+              if constexpr (DIM == 2) {
+                if (i == j) kij = a.c0 * gA[i] * gB[i] + a.c2 * gA[1 - i] * gB[1 - i];
+                else kij = a.c1 * gA[i] * gB[j] + a.c2 * gA[j] * gB[i];
+              } else {
+                if (i == j) kij = a.c0 * gA[i] * gB[i] + a.c2 * (dot - gA[i] * gB[i]);
+                else kij = a.c1 * gA[i] * gB[j] + a.c2 * gA[j] * gB[i];
+              }
+              kl[b][r][j] += sc * kij;
+            }
+          }
+        }
+      }
+      const uint8_t* sl = a.slot + (e * NEN + la) * NEN;
+#pragma unroll
+      for (int b = 0; b < NEN; ++b) {
+        const int s = sl[b];
+#pragma unroll
+        for (int r = 0; r < RPT; ++r)
+#pragma unroll
+          for (int j = 0; j < DIM; ++j) s_acc[((s * RPT + r) * DIM + j) * 64 + lane] += kl[b][r][j];
+      }
+    }
+    if (live) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        const int64_t row = v * DIM + r0 + r;
+        const int32_t ns = ga.rownnz[row] / DIM;
+        for (int s = 0; s < ns; ++s)
+#pragma unroll
+          for (int j = 0; j < DIM; ++j)
+            a.val[row + (int64_t(s) * DIM + j) * a.n] = s_acc[((s * RPT + r) * DIM + j) * 64 + lane];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // ELL expansion: row dim*v+c gets columns dim*w+c' for w in adj(v).
 // ---------------------------------------------------------------------------
 __global__ void k_ell_expand(int64_t n, int32_t width, int dim, const int64_t* adj_ptr,
@@ -681,6 +815,60 @@ hipError_t launch_assembly_color(hipStream_t s, const AssemblyArgs& a) {
     case GF_Q8: return g3 ? launch_asm<2, 8, 9>(s, k, v) : launch_asm<2, 8, 4>(s, k, v);
     case GF_BRICK8: return g3 ? launch_asm<3, 8, 27>(s, k, v) : launch_asm<3, 8, 8>(s, k, v);
     case GF_BRICK20: return g3 ? launch_asm<3, 20, 27>(s, k, v) : launch_asm<3, 20, 8>(s, k, v);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+namespace {
+template <int DIM, int NEN, int NGP, int MAXS>
+hipError_t launch_gather(hipStream_t s, const GArgs& g) {
+  constexpr int RPT = 1;  // one row per thread: the LDS accumulators bound the occupancy
+  const int64_t units = g.nnodes * (DIM / RPT);
+  if (units == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_assemble_gather<DIM, NEN, NGP, MAXS, RPT>),
+                     dim3(grid_for(units, 64, 65536)), dim3(64), 0, s, g);
+  return hipGetLastError();
+}
+}  // namespace
+
+int gather_max_slots(int type) {
+  switch (type) {
+    case GF_Q4: return 9;
+    case GF_Q8: return 21;
+    case GF_BRICK8: return 27;
+    default: return 0;  // Brick20: 81 slots x 3 doubles per row exceed the LDS budget
+  }
+}
+
+hipError_t launch_assembly_gather(hipStream_t s, const AssemblyArgs& a) {
+  ElemInfo ei;
+  if (!elem_info(a.type, &ei) || (a.ngauss != 2 && a.ngauss != 3) || gather_max_slots(a.type) == 0)
+    return hipErrorInvalidValue;
+  GArgs g;
+  KArgs& k = g.k;
+  k.conn = a.conn;
+  k.coords = a.coords;
+  k.elemE = a.elemE;
+  k.E = a.E;
+  k.c0 = a.c[0];
+  k.c1 = a.c[1];
+  k.c2 = a.c[2];
+  k.elems = nullptr;
+  k.nce = 0;
+  k.slot = a.slot;
+  k.val = a.val;
+  k.n = a.n;
+  k.bad = a.bad_jac;
+  k.tab = h_off[tab_block(a.type, a.ngauss)];
+  g.n2e_ptr = a.n2e_ptr;
+  g.n2e = a.n2e;
+  g.rownnz = a.rownnz;
+  g.nnodes = a.nnodes;
+  const bool g3 = a.ngauss == 3;
+  switch (a.type) {
+    case GF_Q4: return g3 ? launch_gather<2, 4, 9, 9>(s, g) : launch_gather<2, 4, 4, 9>(s, g);
+    case GF_Q8: return g3 ? launch_gather<2, 8, 9, 21>(s, g) : launch_gather<2, 8, 4, 21>(s, g);
+    case GF_BRICK8: return g3 ? launch_gather<3, 8, 27, 27>(s, g) : launch_gather<3, 8, 8, 27>(s, g);
     default: return hipErrorInvalidValue;
   }
 }

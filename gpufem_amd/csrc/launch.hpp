@@ -32,9 +32,21 @@ struct AssemblyArgs {
   int64_t n;
   int* bad_jac;         // set to 1 when some detJ <= 0
   int variant;          // 0: element-major threads, 1: pair-major threads,
-                        // 2: FP64 matrix cores (3D elements)
+                        // 2: FP64 matrix cores (3D elements), 3: node gather
+  // node gather: node -> (element << 5 | local node) lists, ascending element
+  const int64_t* n2e_ptr;
+  const int32_t* n2e;
+  const int32_t* rownnz;
+  int64_t nnodes;
 };
 hipError_t launch_assembly_color(hipStream_t s, const AssemblyArgs& a);
+// Node-gather assembly (variant 3): one launch for the whole mesh, no
+// coloring.  Each thread owns rows of one node, computes the rows of Ke of
+// every element around the node and adds them in LDS, then writes its rows
+// of the ELL values once (every live slot).  Applicable when the node has at
+// most gather_max_slots(type) neighbours (0: not available for the type).
+int gather_max_slots(int type);
+hipError_t launch_assembly_gather(hipStream_t s, const AssemblyArgs& a);
 
 // ---- penalty -------------------------------------------------------------------
 // result[0] <- coef * max_i K_ii (0 for rows with no stored diagonal)

@@ -57,16 +57,19 @@ def _solve(ctx, K, d_b, d_x, nrhs=1, **kw):
 def _assembly(p, K, ta, ctx, mesh):
     """Numeric assembly time of each kernel variant (3D: also the FP64
     matrix-core one); the matrix is left assembled by the fastest of the
-    bitwise-equal variants 0/1, with the penalty applied."""
+    tuned variants (0, 1 and, where it fits, the node gather 3), with the
+    penalty applied."""
     chosen, ms01 = ctx.assemble_tune(mesh, K)
     variants = {"element_major": float(ms01[0]), "pair_major": float(ms01[1])}
+    if ms01[3] > 0:
+        variants["node_gather"] = float(ms01[3])
     if p.dim == 3:
         ctx.assemble_tune(mesh, K, 2)
         variants["matrix_core"] = min(ctx.assemble(mesh, K).ms_device for _ in range(2))
         ctx.assemble_tune(mesh, K, chosen)
     t2 = ctx.assemble(mesh, K)
     ctx.apply_penalty(K, p.fixed_dofs)
-    ms = min(ta.ms_device, t2.ms_device, *ms01)
+    ms = min(ta.ms_device, t2.ms_device, *[v for v in ms01 if v > 0])
     return {"ms": ms, "dof_per_s": p.ndof / (ms * 1e-3), "elements": int(p.conn.shape[0]),
             "variant": chosen, "variant_ms": variants}
 

@@ -172,18 +172,25 @@ int gf_ell_download(gf_ctx* ctx, const gf_ell* m, double* val, int32_t* col,
                     int32_t* rownnz);
 
 /* Numeric assembly: Ke by Gauss quadrature fused with the colored,
- * atomic-free scatter (one launch per color).  Values are zeroed first. */
+ * atomic-free scatter (one launch per color), or with the node gather
+ * (variant 3, one launch); every stored value is recomputed. */
 int gf_assemble(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, gf_timing* t);
 /* Assembly kernel of this matrix: 0 = one thread per (element, node pair)
  * with the element outermost; 1 = pair-major, consecutive threads on the same
  * node pair of consecutive elements of a color (neighbouring rows in the
  * scatter); 2 (Brick8/Brick20 only, forced only) = one wavefront per element
  * forming Ke = sum_g B^T (D B) w detJ E as one GEMM on the FP64 matrix
- * cores.  0 and 1 add the same terms in the same order (bitwise-identical
- * values); 2 sums in another order.  force_variant 0/1/2 fixes it; -1
- * assembles twice with 0 and 1 and keeps the faster (gf_assemble does this on
- * its first call for meshes of 65536 elements or more).  ms_per_variant (may
- * be NULL): 2 averages. */
+ * cores; 3 (Q4, Q8, Brick8 with at most 9 / 21 / 27 neighbours per node) =
+ * node gather, one thread per node row(s) forming the node's rows of every
+ * surrounding element's Ke and writing its ELL rows once (no coloring, no
+ * read-modify-write).  0 and 1 add the same terms in the same order
+ * (bitwise-identical values); 2 and 3 sum in other orders (equal to
+ * rounding).  force_variant 0..3 fixes it (GF_ERR_UNSUPPORTED where 3 does
+ * not fit); -1 assembles twice with each of 0, 1 and 3 (where it fits) and
+ * keeps the fastest (gf_assemble does this on its first call for meshes of
+ * 65536 elements or more).  ms_per_variant (may be NULL): GF_ASM_VARIANTS
+ * entries, the better of two timings of each variant timed, -1 otherwise. */
+#define GF_ASM_VARIANTS 4
 int gf_assemble_tune(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int32_t force_variant,
                      double* ms_per_variant, int32_t* chosen);
 

@@ -63,6 +63,7 @@ def test_python_constants_match_header():
     text = open(os.path.join(ROOT, "include", "gpufem.h")).read()
     defines = dict(re.findall(r"#define (GF_[A-Z0-9_]+) \(?(-?\d+)\)?", text))
     assert int(defines["GF_SPMV_VARIANTS"]) == _lib.GF_SPMV_VARIANTS
+    assert int(defines["GF_ASM_VARIANTS"]) == _lib.GF_ASM_VARIANTS
     for name in ("GF_OK", "GF_ERR_ARG", "GF_ERR_NOCONV", "GF_Q4", "GF_BRICK20", "GF_PREC_MIXED",
                  "GF_SCHED_PERSISTENT"):
         assert int(defines[name]) == getattr(_lib, name), name
