@@ -82,6 +82,7 @@ __global__ __launch_bounds__(64) void k_adjacency(const int32_t* __restrict__ co
   __shared__ int32_t s_c[kMaxCand];
   __shared__ uint8_t s_first[kMaxCand];
   const int lane = threadIdx.x;
+  int32_t my_max = 0;  // this workgroup's largest count (one atomic at the end)
   for (int64_t v = blockIdx.x; v < nnodes; v += gridDim.x) {
     const int64_t p0 = n2e_ptr[v];
     const int m = int((n2e_ptr[v + 1] - p0) * nen);
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(64) void k_adjacency(const int32_t* __restrict__ co
         if (FILL) adj[adj_ptr[v]] = int32_t(v);
         else {
           cnt[v] = 1;
-          atomicMax(maxcnt, 1);
+          my_max = my_max > 1 ? my_max : 1;
         }
       }
       continue;
@@ -128,11 +129,12 @@ __global__ __launch_bounds__(64) void k_adjacency(const int32_t* __restrict__ co
       for (int off = 32; off > 0; off >>= 1) local += __shfl_xor(local, off);
       if (lane == 0) {
         cnt[v] = local;
-        atomicMax(maxcnt, local);
+        my_max = my_max > local ? my_max : local;
       }
     }
     __syncthreads();  // LDS reused by the next node
   }
+  if (!FILL && lane == 0 && my_max > 0) atomicMax(maxcnt, my_max);
 }
 
 // slot[e][a][b] = rank of node b in the sorted adjacency of node a
