@@ -164,6 +164,7 @@ __global__ void k_color_round(const int32_t* __restrict__ conn, int64_t nelem, i
                               const int64_t* __restrict__ n2e_ptr,
                               const int32_t* __restrict__ n2e, int32_t* color, int32_t* pending,
                               int32_t* err) {
+  bool pend = false;  // some element of this thread waits for a neighbour
   for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < nelem;
        e += int64_t(gridDim.x) * blockDim.x) {
     if (__hip_atomic_load(color + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) continue;
@@ -183,7 +184,7 @@ __global__ void k_color_round(const int32_t* __restrict__ conn, int64_t nelem, i
       }
     }
     if (!ready) {
-      atomicAdd(pending, 1);
+      pend = true;
       continue;
     }
     if (used == ~0ull) {
@@ -193,6 +194,10 @@ __global__ void k_color_round(const int32_t* __restrict__ conn, int64_t nelem, i
     const int32_t c = __ffsll(static_cast<long long>(~used));  // 1-based lowest free
     __hip_atomic_store(color + e, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+  // one store per wavefront with a waiting element (pending was zeroed
+  // before the round; every writer writes 1)
+  const int any = __any(pend);
+  if (any && (threadIdx.x & 63) == 0) *pending = 1;
 }
 
 __global__ void k_iota(int32_t* p, int64_t n) {
