@@ -280,6 +280,15 @@ class Context:
         self._check(self.lib.gf_profile_spmv(self.handle, ell.handle, nrep, C.byref(ms)))
         return ms.value
 
+    def profile_pcg(self, ell, b, iters=200):
+        """Average SpMV kernel time and iteration time (ms) over iters PCG
+        iterations from zero for b (events around each SpMV)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        ms_spmv, ms_it = C.c_double(), C.c_double()
+        self._check(self.lib.gf_profile_pcg(self.handle, ell.handle, _ptr(b), iters,
+                                            C.byref(ms_spmv), C.byref(ms_it)))
+        return ms_spmv.value, ms_it.value
+
     def pcg(self, ell, b, x0=None, raise_on_fail=True, **opts):
         """Solve K x = b; b is (n,) or (nrhs, n).  Returns (x, SolveStats)."""
         b = np.ascontiguousarray(b, dtype=np.float64)

@@ -24,7 +24,7 @@ GF_Q4, GF_Q8, GF_BRICK8, GF_BRICK20 = 1, 2, 3, 4
 GF_PRECOND_NONE, GF_PRECOND_JACOBI = 0, 1
 GF_PREC_FP64, GF_PREC_MIXED = 0, 1
 GF_SCHED_AUTO, GF_SCHED_KERNELS, GF_SCHED_GRAPH, GF_SCHED_PERSISTENT = 0, 1, 2, 3
-GF_SPMV_VARIANTS = 9
+GF_SPMV_VARIANTS = 11
 GF_ASM_VARIANTS = 4  # entries gf_assemble_tune writes into ms_per_variant
 
 
@@ -99,6 +99,8 @@ SIGNATURES = {
     "gf_pcg": (C.c_int, [_P, _P, _P, _P, _I32, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "gf_ell_tune": (C.c_int, [_P, _P, _I32, _P, _P]),
     "gf_profile_spmv": (C.c_int, [_P, _P, _I32, C.POINTER(C.c_double)]),
+    "gf_profile_pcg": (C.c_int, [_P, _P, _P, _I32, C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double)]),
     "gf_stream_copy": (C.c_int, [_P, _P, _P, _I64, _I32, C.POINTER(C.c_double)]),
     "gf_group_create": (C.c_int, [_P, _I32, C.POINTER(MeshDesc), C.POINTER(_P)]),
     "gf_group_info": (C.c_int, [_P, _P, _P, _P]),

@@ -26,35 +26,52 @@ __device__ inline double block_sum(double v, double* sh) {
   return t;
 }
 
-// Sum of np per-block partials, identical in every workgroup.
+// Sum of np per-block partials, identical in every workgroup (a fixed order:
+// four strided accumulators per thread, so four loads are in flight when the
+// SpMV grid leaves thousands of partials).
 __device__ inline double reduce_partials(const double* part, int np, double* sh) {
-  double v = 0.0;
-  for (int i = threadIdx.x; i < np; i += kNT) v += part[i];
-  return block_sum(v, sh);
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  int i = threadIdx.x;
+  for (; i + 3 * kNT < np; i += 4 * kNT) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] += part[i + u * kNT];
+  }
+  // at most three left
+  if (i < np) v[0] += part[i];
+  if (i + kNT < np) v[1] += part[i + kNT];
+  if (i + 2 * kNT < np) v[2] += part[i + 2 * kNT];
+  return block_sum((v[0] + v[1]) + (v[2] + v[3]), sh);
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs; the remap gives
 // XCD k the logical blocks [k*nb/8, (k+1)*nb/8), i.e. a contiguous slab of
 // rows, so gathers of neighbouring rows share that XCD's L2.
-__device__ inline int logical_block() {
-  const int nb = gridDim.x, b = blockIdx.x;
+__device__ inline int xcd_block(int b, int nb) {
   if ((nb & 7) != 0) return b;
   return (b & 7) * (nb >> 3) + (b >> 3);
 }
+__device__ inline int logical_block() { return xcd_block(blockIdx.x, gridDim.x); }
 
 // Rows are visited in tiles of kNT consecutive rows; tile t belongs to
 // logical workgroup t mod gridDim.x (grid-stride sweeps).  At any moment
 // the whole grid works inside one window of gridDim.x tiles, which keeps
 // the slot-major ELL streams DRAM-page friendly, and each XCD owns a
-// contiguous part of every window.  Every PCG kernel uses this ownership,
-// so a row's vector entries are written and re-read on the same XCD.
+// contiguous part of every window.  The PCG vector kernels use this
+// ownership (the slice-group SpMV grids own one slab of rows per XCD instead;
+// giving the vector kernels the same slabs measured no faster on the Q8
+// plate).  for_rows_of: the rows of logical workgroup lb of nlb (a
+// persistent kernel plays several logical workgroups in turn).
 template <class F>
-__device__ inline void for_my_rows(int64_t n, F&& f) {
-  const int64_t stride = int64_t(gridDim.x) * kNT;
-  for (int64_t base = int64_t(logical_block()) * kNT; base < n; base += stride) {
+__device__ inline void for_rows_of(int lb, int nlb, int64_t n, F&& f) {
+  const int64_t stride = int64_t(nlb) * kNT;
+  for (int64_t base = int64_t(lb) * kNT; base < n; base += stride) {
     const int64_t i = base + threadIdx.x;
     if (i < n) f(i);
   }
+}
+template <class F>
+__device__ inline void for_my_rows(int64_t n, F&& f) {
+  for_rows_of(logical_block(), gridDim.x, n, f);
 }
 
 // One row of a slot-major ELL matrix, U slots in flight; slots are summed in
@@ -143,19 +160,24 @@ __device__ inline void load_node(const T* __restrict__ x, int64_t c, T* xv) {
   }
 }
 
-// Node row R of the block-sliced copy, lane l of slice s: acc[r] = (K x)_{dR+r}.
+// Node row R of the block-sliced copy, lane l of slice s: acc[r] = (K x)_{dR+r}
+// over cnt block slots from b0 (the row's blocks, then zero padding).
 // Entries are added in ascending column order, as row_dot adds its slots.
+// U blocks per step (measured on the Q8 plate and the Brick8 cube: 4 for
+// 2 x 2 blocks, 3 for 3 x 3; fewer lose more in-flight loads than the extra
+// occupancy gives back, more cost occupancy).
 template <int D, class T>
 __device__ inline void bsell_row(const BsellView& bs, const T* __restrict__ bval,
                                  const int32_t* __restrict__ bcol, int64_t b0, int32_t cnt,
                                  int lane, const T* __restrict__ x, T* acc) {
+  constexpr int U = D == 2 ? 4 : 3;
 #pragma unroll
   for (int r = 0; r < D; ++r) acc[r] = T(0);
   int32_t j = 0;
-  for (; j + 2 <= cnt; j += 2) {
-    T xv[2][D], v[2][D * D];
+  for (; j + U <= cnt; j += U) {
+    T xv[U][D], v[U][D * D];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < U; ++u) {
       const int64_t b = b0 + j + u;
       const int64_t c = int64_t(__builtin_nontemporal_load(bcol + b * 64 + lane)) * D;
 #pragma unroll
@@ -164,13 +186,13 @@ __device__ inline void bsell_row(const BsellView& bs, const T* __restrict__ bval
       load_node<D, T>(x, c, xv[u]);
     }
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int u = 0; u < U; ++u)
 #pragma unroll
       for (int r = 0; r < D; ++r)
 #pragma unroll
         for (int e = 0; e < D; ++e) acc[r] += v[u][r * D + e] * xv[u][e];
   }
-  if (j < cnt) {
+  for (; j < cnt; ++j) {
     const int64_t b = b0 + j;
     const int64_t c = int64_t(__builtin_nontemporal_load(bcol + b * 64 + lane)) * D;
     T xv[D];
@@ -183,43 +205,49 @@ __device__ inline void bsell_row(const BsellView& bs, const T* __restrict__ bval
   }
 }
 
-// Calls f(i, (K x)_i) for every row of this workgroup's range, with the
-// SpMV variant V (thread per row with 1/4/8 slots in flight, or 2/4 waves
-// sharing a tile of rows).  Every thread of the workgroup must call it.
+// Calls f(i, (K x)_i) for every row of logical workgroup lb of nlb, with
+// the SpMV variant V (thread per row with 1/4/8 slots in flight, 2/4 waves
+// sharing a tile of rows, node blocks, block-sliced slices).  Every thread of
+// the workgroup must call it.
 template <int V, class T, class F>
-__device__ inline void for_rows_spmv(int64_t n, const T* __restrict__ val,
+__device__ inline void for_rows_spmv(int lb, int nlb, int64_t n, const T* __restrict__ val,
                                      const int32_t* __restrict__ col,
                                      const int32_t* __restrict__ rownnz,
                                      const T* __restrict__ x, F&& f, const BsellView& bs) {
-  if constexpr (V == kBsell2 || V == kBsell3) {
-    // four slices (one per wave) per workgroup and sweep, grid-stride
-    constexpr int D = V == kBsell2 ? 2 : 3;
+  if constexpr (V == kBsell2 || V == kBsell3 || V == kBsell2G || V == kBsell3G) {
+    // four slices (one per wave) per workgroup and sweep, grid-stride (one
+    // sweep when the grid has a workgroup per group)
+    constexpr int D = (V == kBsell2 || V == kBsell2G) ? 2 : 3;
     const int lane = threadIdx.x & 63;
-    const int64_t stride = int64_t(gridDim.x) * (kNT / 64);
-    for (int64_t sl = int64_t(logical_block()) * (kNT / 64) + (threadIdx.x >> 6); sl < bs.nslices;
+    const int64_t stride = int64_t(nlb) * (kNT / 64);
+    for (int64_t sl = int64_t(lb) * (kNT / 64) + (threadIdx.x >> 6); sl < bs.nslices;
          sl += stride) {
+      // every lane runs the slice's length: the loads start without waiting
+      // for the row number, and padding blocks add zeros (x finite: the same
+      // sums, up to the sign of a zero sum)
+      const int64_t b0 = bs.sptr[sl];
+      const int32_t w = int32_t(bs.sptr[sl + 1] - b0);
+      T acc[D];
+      bsell_row<D, T>(bs, val, col, b0, w, lane, x, acc);
       const int32_t R = bs.perm[sl * 64 + lane];
       if (R < 0) continue;
-      const int32_t cnt = rownnz[int64_t(D) * R] / D;
-      T acc[D];
-      bsell_row<D, T>(bs, val, col, bs.sptr[sl], cnt, lane, x, acc);
 #pragma unroll
       for (int r = 0; r < D; ++r) f(int64_t(D) * R + r, acc[r]);
     }
   } else if constexpr (V == kNode2 || V == kNode3) {
     constexpr int D = V == kNode2 ? 2 : 3;
-    for_my_rows(n, [&](int64_t i) { f(i, row_dot_node<D, T>(i, n, val, col, rownnz[i], x)); });
+    for_rows_of(lb, nlb, n, [&](int64_t i) { f(i, row_dot_node<D, T>(i, n, val, col, rownnz[i], x)); });
   } else if constexpr (V <= kThreadU8) {
     constexpr int U = V == kThreadU1 ? 1 : (V == kThreadU4 ? 4 : 8);
-    for_my_rows(n, [&](int64_t i) { f(i, row_dot<U, T>(i, n, val, col, rownnz[i], x)); });
+    for_rows_of(lb, nlb, n, [&](int64_t i) { f(i, row_dot<U, T>(i, n, val, col, rownnz[i], x)); });
   } else {
     // the workgroup's kNT-row tile is covered in SUB passes of RPB rows
     constexpr int SUB = V == kTile2 ? 2 : 4;
     constexpr int RPB = kNT / SUB;  // rows per pass
     __shared__ T sh_part[SUB][RPB];
     const int rl = threadIdx.x % RPB, sub = threadIdx.x / RPB;
-    const int64_t stride = int64_t(gridDim.x) * kNT;
-    for (int64_t tile = int64_t(logical_block()) * kNT; tile < n; tile += stride) {
+    const int64_t stride = int64_t(nlb) * kNT;
+    for (int64_t tile = int64_t(lb) * kNT; tile < n; tile += stride) {
       for (int pass = 0; pass < SUB; ++pass) {
         const int64_t i = tile + pass * RPB + rl;
         T acc = T(0);
@@ -287,6 +315,12 @@ __device__ inline void for_rows_spmv(int64_t n, const T* __restrict__ val,
       break;                                                                              \
     case ::gf::kBsell3:                                                                   \
       hipLaunchKernelGGL(KERNEL<::gf::kBsell3>, GRID, dim3(::gf::dev::kNT), 0, STREAM, __VA_ARGS__); \
+      break;                                                                              \
+    case ::gf::kBsell2G:                                                                  \
+      hipLaunchKernelGGL(KERNEL<::gf::kBsell2G>, GRID, dim3(::gf::dev::kNT), 0, STREAM, __VA_ARGS__); \
+      break;                                                                              \
+    case ::gf::kBsell3G:                                                                  \
+      hipLaunchKernelGGL(KERNEL<::gf::kBsell3G>, GRID, dim3(::gf::dev::kNT), 0, STREAM, __VA_ARGS__); \
       break;                                                                              \
     default:                                                                              \
       return hipErrorInvalidValue;                                                        \

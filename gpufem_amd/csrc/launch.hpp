@@ -79,14 +79,21 @@ hipError_t launch_load_vector(hipStream_t s, int dim, int64_t n, const double* l
 // column array of gf_ell::d_coln) and the d entries of x of that node together.
 // kBsell2 / kBsell3: thread per node row of the block-sliced copy (BsellView;
 // "val"/"col" are then gf_ell::d_bval / d_bcol): d output rows per thread,
-// one node column and one gather of the node's d x entries per d x d block.
+// one node column and one gather of the node's d x entries per d x d block;
+// the PCG grid sweeps the slices.
+// kBsell2G / kBsell3G: the same rows, one group of four slices (one per wave)
+// per workgroup and as many workgroups as groups, so that the hardware
+// dispatcher balances slices of unequal length.
 enum SpmvVariant {
   kThreadU1 = 0, kThreadU4 = 1, kThreadU8 = 2, kTile2 = 3, kTile4 = 4, kNode2 = 5, kNode3 = 6,
-  kBsell2 = 7, kBsell3 = 8
+  kBsell2 = 7, kBsell3 = 8, kBsell2G = 9, kBsell3G = 10
 };
-constexpr int kNumSpmvVariants = 9;
+constexpr int kNumSpmvVariants = 11;
 inline bool is_node_variant(int v) { return v == kNode2 || v == kNode3; }
-inline bool is_bsell_variant(int v) { return v == kBsell2 || v == kBsell3; }
+inline bool is_bsell_group_variant(int v) { return v == kBsell2G || v == kBsell3G; }
+inline bool is_bsell_variant(int v) {
+  return v == kBsell2 || v == kBsell3 || is_bsell_group_variant(v);
+}
 // node columns: coln[i + j n] = col[i + d j n] / d for j < width / d
 hipError_t launch_build_coln(hipStream_t s, int64_t n, int32_t width, int d, const int32_t* col,
                              int32_t* coln);
@@ -110,9 +117,14 @@ hipError_t launch_bsell_cols(hipStream_t s, const BsellView& bs, int d, int64_t 
 hipError_t launch_bsell_vals(hipStream_t s, const BsellView& bs, int d, int64_t n,
                              const double* val, const int32_t* rownnz, double* bval);
 constexpr int kMaxPcgBlocks = 2048;
+// SpMV workgroups of the slice-group variants (one partial p.q each)
+constexpr int kMaxSpmvBlocks = 65536;
 
 // workgroups of the standalone SpMV and the PCG kernels for n rows
 int spmv_blocks(int64_t n);
+// workgroups of the SpMV kernels of `variant`: one per four-slice group for
+// the slice-group variants, else `nblocks`
+int spmv_grid(int variant, const BsellView& bs, int nblocks);
 // bs: the block-sliced view (used by the kBsell variants only; may be null)
 hipError_t launch_spmv(hipStream_t s, int variant, int64_t n, const double* val,
                        const int32_t* col, const int32_t* rownnz, const double* x, double* y,
@@ -138,9 +150,10 @@ struct PcgBuffers {
   double* z;
   double* p;
   double* q;
-  double* part1;  // per-block partials of p.q
+  double* part1;  // per-SpMV-workgroup partials of p.q
   double* part2;  // per-block partials of (r.z, r.r[, b.b])
-  int nblocks;    // workgroups of every PCG kernel (<= kMaxPcgBlocks)
+  int nblocks;    // workgroups of every PCG kernel but the SpMV (<= kMaxPcgBlocks)
+  int spmv_grid;  // workgroups of the SpMV kernel (<= kMaxSpmvBlocks)
   PcgState* st;
   BsellView bs;   // block-sliced view (kBsell variants)
 };
@@ -149,6 +162,9 @@ size_t grid_barrier_bytes();
 hipError_t launch_pcg_setup(hipStream_t s, const PcgBuffers& B, int use_x0, double rtol,
                             int maxit);
 hipError_t launch_pcg_iteration(hipStream_t s, const PcgBuffers& B, int parity);
+// the same, with events recorded around the SpMV kernel
+hipError_t launch_pcg_iteration_timed(hipStream_t s, const PcgBuffers& B, int parity,
+                                      hipEvent_t spmv_start, hipEvent_t spmv_stop);
 // co-resident workgroups for the persistent kernel of this variant (0: unavailable)
 int persistent_blocks(int variant, int num_cu);
 // all remaining iterations in one cooperative launch

@@ -121,6 +121,7 @@ int run_mixed(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, PcgHostState* h
   B.n = n;
   B.variant = P->variant;
   B.nblocks = P->nblocks;
+  B.spmv_grid = spmv_grid(P->variant, o.bs, P->nblocks);
   B.val = o.val;
   B.col = o.col;
   B.bs = o.bs;

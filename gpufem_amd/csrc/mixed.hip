@@ -102,10 +102,11 @@ __global__ __launch_bounds__(kNT) void k_m_spmv_pq(MixedBuffers B) {
   if (B.st->done) return;
   __shared__ double sh[kNT / 64];
   double pq = 0.0;
-  for_rows_spmv<V, float>(B.n, B.val32, B.col, B.rownnz, B.p32, [&](int64_t i, float qi) {
-    B.q32[i] = qi;
-    pq += double(B.p32[i]) * double(qi);
-  }, B.bs);
+  for_rows_spmv<V, float>(logical_block(), gridDim.x, B.n, B.val32, B.col, B.rownnz, B.p32,
+                          [&](int64_t i, float qi) {
+                            B.q32[i] = qi;
+                            pq += double(B.p32[i]) * double(qi);
+                          }, B.bs);
   pq = block_sum(pq, sh);
   if (threadIdx.x == 0) B.part1[blockIdx.x] = pq;
 }
@@ -114,7 +115,7 @@ __global__ __launch_bounds__(kNT) void k_m_update(MixedBuffers B, int parity) {
   MixedState* st = B.st;
   if (st->done) return;
   __shared__ double sh[kNT / 64];
-  const double pq = reduce_partials(B.part1, gridDim.x, sh);
+  const double pq = reduce_partials(B.part1, B.spmv_grid, sh);
   if (!(pq > 0.0)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->status = GF_ERR_BREAKDOWN;
@@ -160,7 +161,8 @@ __global__ __launch_bounds__(kNT) void k_m_refresh(MixedBuffers B, int parity) {
   double rz, rr;
   if (!refresh_due(B, parity, &rz, &rr, sh)) return;
   double rz2 = 0.0, rr2 = 0.0;
-  for_rows_spmv<V, double>(B.n, B.val, B.col, B.rownnz, B.x, [&](int64_t i, double kx) {
+  for_rows_spmv<V, double>(logical_block(), gridDim.x, B.n, B.val, B.col, B.rownnz, B.x,
+                           [&](int64_t i, double kx) {
     const double ri = B.b[i] - kx;
     const float r32 = float(ri);
     const float z32 = B.dinv32[i] * r32;
@@ -243,7 +245,7 @@ hipError_t launch_mixed_setup(hipStream_t s, const MixedBuffers& B, int use_x0, 
 hipError_t launch_mixed_iteration(hipStream_t s, const MixedBuffers& B, int parity) {
   hipError_t e;
   const dim3 g(B.nblocks);
-  GF_DISPATCH_VARIANT(B.variant, k_m_spmv_pq, g, s, B);
+  GF_DISPATCH_VARIANT(B.variant, k_m_spmv_pq, dim3(B.spmv_grid), s, B);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_m_update, g, dim3(kNT), 0, s, B, parity);
   if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -13,7 +13,8 @@ struct MixedState;
 struct MixedBuffers {
   int64_t n;
   int variant;
-  int nblocks;
+  int nblocks;    // workgroups of every kernel but the FP32 SpMV
+  int spmv_grid;  // workgroups of the FP32 SpMV (slice-group variants: one per group)
   // FP64 outer system
   const double* val;
   const int32_t* col;

@@ -73,7 +73,7 @@ int ensure_plan(gf_ctx* ctx, gf_ell* m) {
   double** vecs[] = {&P->r, &P->z, &P->p, &P->q, &P->wb, &P->wx};
   for (double** v : vecs)
     if (e == hipSuccess) e = hipMalloc(v, vb);
-  if (e == hipSuccess) e = hipMalloc(&P->part1, kMaxPcgBlocks * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&P->part1, kMaxSpmvBlocks * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&P->part2, 3 * kMaxPcgBlocks * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&P->res, 64 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&P->st), pcg_state_bytes());
@@ -119,6 +119,8 @@ int bsell_variant(const gf_ell* m) {
 bool variant_fits(const gf_ell* m, int v) {
   if (v < 0 || v >= kNumSpmvVariants) return false;
   if (is_node_variant(v)) return v == node_variant(m);
+  if (v == kBsell2G) return bsell_variant(m) == kBsell2;
+  if (v == kBsell3G) return bsell_variant(m) == kBsell3;
   if (is_bsell_variant(v)) return v == bsell_variant(m);
   return true;
 }
@@ -227,6 +229,7 @@ PcgBuffers make_buffers(gf_ell* m, const double* dinv) {
   B.part1 = P->part1;
   B.part2 = P->part2;
   B.nblocks = P->nblocks;
+  B.spmv_grid = spmv_grid(P->variant, o.bs, P->nblocks);
   B.st = P->st;
   return B;
 }
@@ -582,6 +585,60 @@ int gf_profile_spmv(gf_ctx* ctx, gf_ell* m, int32_t nrep, double* ms_avg) {
   float ms = 0.f;
   GF_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
   *ms_avg = double(ms) / nrep;
+  return GF_OK;
+}
+
+int gf_profile_pcg(gf_ctx* ctx, gf_ell* m, const double* b, int32_t iters, double* ms_spmv,
+                   double* ms_iteration) {
+  if (!ctx || !m || !b || iters < 1 || !ms_spmv || !ms_iteration)
+    return set_error(ctx, GF_ERR_ARG, "gf_profile_pcg: bad argument");
+  GF_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  int st = ensure_plan(ctx, m);
+  if (st) return st;
+  gf_solve_plan* P = m->plan;
+  if (P->variant < 0 && (st = tune_spmv(ctx, m, -1, nullptr, nullptr))) return st;
+  if ((st = prepare_jacobi(ctx, m))) return st;
+  if ((st = prepare_variant(ctx, m, P->variant))) return st;
+  GF_HIP_CHECK(ctx, hipMemcpyAsync(P->wb, b, size_t(m->n) * sizeof(double),
+                                   hipMemcpyHostToDevice, ctx->stream));
+  const PcgBuffers B = make_buffers(m, m->d_dinv);
+  // rtol 0: every one of the iters iterations runs (unless the residual
+  // vanishes exactly)
+  GF_HIP_CHECK(ctx, launch_pcg_setup(ctx->stream, B, 0, 0.0, iters));
+  struct Events {
+    std::vector<hipEvent_t> ev;
+    ~Events() {
+      for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    }
+  } E;
+  E.ev.assign(2 * kChunk, nullptr);
+  for (hipEvent_t& e : E.ev) GF_HIP_CHECK(ctx, hipEventCreate(&e));
+  double spmv = 0.0;
+  GF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
+  for (int done = 0; done < iters;) {
+    const int c = std::min(kChunk, iters - done);
+    for (int k = 0; k < c; ++k)
+      GF_HIP_CHECK(ctx, launch_pcg_iteration_timed(ctx->stream, B, (done + k) & 1, E.ev[2 * k],
+                                                   E.ev[2 * k + 1]));
+    GF_HIP_CHECK(ctx, hipEventSynchronize(E.ev[2 * c - 1]));
+    for (int k = 0; k < c; ++k) {
+      float ms = 0.f;
+      GF_HIP_CHECK(ctx, hipEventElapsedTime(&ms, E.ev[2 * k], E.ev[2 * k + 1]));
+      spmv += ms;
+    }
+    done += c;
+  }
+  GF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
+  GF_HIP_CHECK(ctx, hipEventSynchronize(ctx->ev_stop));
+  float total = 0.f;
+  GF_HIP_CHECK(ctx, hipEventElapsedTime(&total, ctx->ev_start, ctx->ev_stop));
+  PcgHostState h;
+  GF_HIP_CHECK(ctx, read_pcg_state(ctx->stream, B, &h));
+  if (h.it != iters)
+    return set_error(ctx, GF_ERR_BREAKDOWN, "gf_profile_pcg: the iteration stopped early");
+  *ms_spmv = spmv / iters;
+  *ms_iteration = double(total) / iters;
   return GF_OK;
 }
 

@@ -16,6 +16,10 @@
 //                    FEM matrix: one node column per d slots and the d
 //                    entries of x read together (fewer loads, 4/d instead of
 //                    4 column bytes per entry); bitwise equal to kThreadU1.
+//   kBsell2/kBsell3  thread per node row of the block-sliced copy, four
+//                    slices per workgroup in grid-stride sweeps.
+//   kBsell2G/3G      the same with one workgroup per four-slice group (one
+//                    sweep; the dispatcher balances unequal slices).
 //
 // CG schedules: three launches per iteration (k_spmv_pq, k_update,
 // k_direction) with all scalars in a device PcgState, so the host can
@@ -23,7 +27,12 @@
 // persistent kernel (k_pcg_persistent) with three grid barriers per
 // iteration.  Every block reduces the per-block partials in the same fixed
 // order, so all blocks take the same convergence decision and results are
-// bitwise reproducible for a given matrix and variant.
+// bitwise reproducible for a given matrix and variant.  The SpMV kernel may
+// run on a grid of its own (the slice-group variants): k_update reduces its
+// spmv_grid p.q partials, and the persistent kernel plays the same logical
+// SpMV workgroups in turn.  No workgroup waits on another inside a kernel
+// but the persistent one: on gfx950 an agent-scope release writes back the
+// XCD's L2, far too costly once per workgroup per SpMV.
 #include "device_common.hpp"
 #include "gf_internal.hpp"
 #include "launch.hpp"
@@ -61,7 +70,8 @@ __global__ __launch_bounds__(kNT) void k_spmv(int64_t n, const double* __restric
                                               const int32_t* __restrict__ rownnz,
                                               const double* __restrict__ x,
                                               double* __restrict__ y, BsellView bs) {
-  for_rows_spmv<V, double>(n, val, col, rownnz, x, [&](int64_t i, double v) { y[i] = v; }, bs);
+  for_rows_spmv<V, double>(logical_block(), gridDim.x, n, val, col, rownnz, x,
+                           [&](int64_t i, double v) { y[i] = v; }, bs);
 }
 
 // block-sliced copy: node columns (padding blocks point at the row's own
@@ -127,10 +137,12 @@ __global__ void k_jacobi_inv(int64_t n, const double* val, const int32_t* diag, 
 // ---------------------------------------------------------------------------
 // PCG phases (shared by the multi-launch and the persistent schedules)
 // ---------------------------------------------------------------------------
+// q = K p over the rows of logical SpMV workgroup lb of nlb; returns this
+// thread's part of p.q
 template <int V>
-__device__ inline double phase_spmv_pq(const PcgBuffers& B) {
+__device__ inline double phase_spmv_pq(const PcgBuffers& B, int lb, int nlb) {
   double pq = 0.0;
-  for_rows_spmv<V, double>(B.n, B.val, B.col, B.rownnz, B.p, [&](int64_t i, double qi) {
+  for_rows_spmv<V, double>(lb, nlb, B.n, B.val, B.col, B.rownnz, B.p, [&](int64_t i, double qi) {
     B.q[i] = qi;
     pq += B.p[i] * qi;
   }, B.bs);
@@ -212,7 +224,7 @@ template <int V>
 __global__ __launch_bounds__(kNT) void k_spmv_pq(PcgBuffers B) {
   if (B.st->done) return;
   __shared__ double sh[kNT / 64];
-  const double pq = block_sum(phase_spmv_pq<V>(B), sh);
+  const double pq = block_sum(phase_spmv_pq<V>(B, logical_block(), gridDim.x), sh);
   if (threadIdx.x == 0) B.part1[blockIdx.x] = pq;
 }
 
@@ -220,7 +232,7 @@ __global__ __launch_bounds__(kNT) void k_update(PcgBuffers B, int parity) {
   PcgState* st = B.st;
   if (st->done) return;
   __shared__ double sh[kNT / 64];
-  const double pq = reduce_partials(B.part1, gridDim.x, sh);
+  const double pq = reduce_partials(B.part1, B.spmv_grid, sh);
   if (!(pq > 0.0)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->status = GF_ERR_BREAKDOWN;
@@ -308,11 +320,15 @@ __global__ __launch_bounds__(kNT) void k_pcg_persistent(PcgBuffers B, GridBarrie
   const double rtol2 = st->rtol2, bnorm2 = st->bnorm2;
   const int maxit = st->maxit;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const int nlb = B.spmv_grid;
   for (;;) {
-    const double pq_part = block_sum(phase_spmv_pq<V>(B), sh);
-    if (threadIdx.x == 0) B.part1[blockIdx.x] = pq_part;
+    // the SpMV kernel's logical workgroups j (same rows, same partial slots)
+    for (int j = blockIdx.x; j < nlb; j += gridDim.x) {
+      const double pq_part = block_sum(phase_spmv_pq<V>(B, xcd_block(j, nlb), nlb), sh);
+      if (threadIdx.x == 0) B.part1[j] = pq_part;
+    }
     if (!grid_barrier(bar, &s_flag)) break;
-    const double pq = reduce_partials(B.part1, gridDim.x, sh);
+    const double pq = reduce_partials(B.part1, nlb, sh);
     if (!(pq > 0.0)) {
       if (lead) {
         st->status = GF_ERR_BREAKDOWN;
@@ -448,12 +464,21 @@ hipError_t launch_bsell_vals(hipStream_t s, const BsellView& bs, int d, int64_t 
   return hipGetLastError();
 }
 
+int spmv_grid(int variant, const BsellView& bs, int nblocks) {
+  if (!is_bsell_group_variant(variant)) return nblocks;
+  int64_t nb = (bs.nslices + kNT / 64 - 1) / (kNT / 64);
+  nb = (nb + 7) & ~int64_t(7);  // the XCD remap needs a multiple of 8
+  if (nb > kMaxSpmvBlocks) nb = kMaxSpmvBlocks;
+  if (nb < 8) nb = 8;
+  return int(nb);
+}
+
 hipError_t launch_spmv(hipStream_t s, int variant, int64_t n, const double* val,
                        const int32_t* col, const int32_t* rownnz, const double* x, double* y,
                        int nrhs, const BsellView* bs) {
-  const dim3 g(spmv_blocks(n));
   const BsellView view = bs ? *bs : BsellView{nullptr, nullptr, 0};
   if (is_bsell_variant(variant) && !view.perm) return hipErrorInvalidValue;
+  const dim3 g(spmv_grid(variant, view, spmv_blocks(n)));
   for (int k = 0; k < nrhs; ++k) {
     const double* xk = x + int64_t(k) * n;
     double* yk = y + int64_t(k) * n;
@@ -487,8 +512,21 @@ hipError_t launch_pcg_setup(hipStream_t s, const PcgBuffers& B, int use_x0, doub
 
 hipError_t launch_pcg_iteration(hipStream_t s, const PcgBuffers& B, int parity) {
   hipError_t e;
-  GF_DISPATCH_VARIANT(B.variant, k_spmv_pq, dim3(B.nblocks), s, B);
+  GF_DISPATCH_VARIANT(B.variant, k_spmv_pq, dim3(B.spmv_grid), s, B);
   if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_update, dim3(B.nblocks), dim3(kNT), 0, s, B, parity);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_direction, dim3(B.nblocks), dim3(kNT), 0, s, B, parity);
+  return hipGetLastError();
+}
+
+hipError_t launch_pcg_iteration_timed(hipStream_t s, const PcgBuffers& B, int parity,
+                                      hipEvent_t spmv_start, hipEvent_t spmv_stop) {
+  hipError_t e = hipEventRecord(spmv_start, s);
+  if (e != hipSuccess) return e;
+  GF_DISPATCH_VARIANT(B.variant, k_spmv_pq, dim3(B.spmv_grid), s, B);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = hipEventRecord(spmv_stop, s)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_update, dim3(B.nblocks), dim3(kNT), 0, s, B, parity);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_direction, dim3(B.nblocks), dim3(kNT), 0, s, B, parity);
@@ -508,6 +546,8 @@ int persistent_blocks(int variant, int num_cu) {
     case kNode3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_persistent<kNode3>, kNT, 0); break;
     case kBsell2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_persistent<kBsell2>, kNT, 0); break;
     case kBsell3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_persistent<kBsell3>, kNT, 0); break;
+    case kBsell2G: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_persistent<kBsell2G>, kNT, 0); break;
+    case kBsell3G: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_persistent<kBsell3G>, kNT, 0); break;
     default: break;
   }
   if (e != hipSuccess || per_cu < 1) return 0;
@@ -535,6 +575,8 @@ hipError_t launch_pcg_persistent(hipStream_t s, const PcgBuffers& B, void* barri
     case kNode3: return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent<kNode3>), g, blk, args, 0, s);
     case kBsell2: return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent<kBsell2>), g, blk, args, 0, s);
     case kBsell3: return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent<kBsell3>), g, blk, args, 0, s);
+    case kBsell2G: return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent<kBsell2G>), g, blk, args, 0, s);
+    case kBsell3G: return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent<kBsell3G>), g, blk, args, 0, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -558,7 +600,7 @@ hipError_t launch_spmv_pq_repeat(hipStream_t s, const PcgBuffers& B, int nrep) {
   hipLaunchKernelGGL(k_state_live, dim3(1), dim3(64), 0, s, B.st);
   hipError_t e = hipGetLastError();
   for (int i = 0; i < nrep && e == hipSuccess; ++i) {
-    GF_DISPATCH_VARIANT(B.variant, k_spmv_pq, dim3(B.nblocks), s, B);
+    GF_DISPATCH_VARIANT(B.variant, k_spmv_pq, dim3(B.spmv_grid), s, B);
     e = hipGetLastError();
   }
   return e;

@@ -292,15 +292,17 @@ int gf_pcg(gf_ctx* ctx, gf_ell* m, const double* b, double* x, int32_t nrhs,
  * a block-sliced copy the solver keeps beside the ELL arrays (node rows
  * sorted by length in windows of 2048, slices of 64, one column index per
  * 2x2 / 3x3 block; rebuilt from the ELL values after each assembly or
- * penalty).  Variants 0..2 and 5..8 add each row's entries in slot order
- * and give bitwise-identical products.  force_variant >= 0 fixes a
+ * penalty), the PCG grid sweeping the slices, 9/10 the same rows with one
+ * workgroup per group of four slices (the hardware dispatcher balances
+ * slices of unequal length).  Variants 0..2 and 5..10 add each row's entries
+ * in slot order and give bitwise-identical products.  force_variant >= 0 fixes a
  * variant; -1 times every variant that fits this matrix and keeps the
  * fastest (gf_pcg does this on first use).  ms_per_variant (may be NULL)
  * receives GF_SPMV_VARIANTS averages (-1 for variants that do not fit).  No
  * reference counterpart; it plays the role of SetOclStrategy /
  * SetOclLocalSize (src/SPRmatrix/SPRmatrix.h:110-121) chosen by measurement
  * instead of by hand. */
-#define GF_SPMV_VARIANTS 9
+#define GF_SPMV_VARIANTS 11
 int gf_ell_tune(gf_ctx* ctx, gf_ell* m, int32_t force_variant, double* ms_per_variant,
                 int32_t* chosen);
 
@@ -310,6 +312,16 @@ int gf_ell_tune(gf_ctx* ctx, gf_ell* m, int32_t force_variant, double* ms_per_va
  * it mirrors the per-kernel event timing of OCLwrapper
  * (src/OpenCL/OCLwrapper.cc:400-434) for the roofline figure. */
 int gf_profile_spmv(gf_ctx* ctx, gf_ell* m, int32_t nrep, double* ms_avg);
+
+/* Runs iters Jacobi-PCG iterations from x = 0 for the right-hand side b
+ * (host array of n) with the three-launch schedule, HIP events around each
+ * iteration's SpMV kernel: ms_spmv is that kernel's average duration inside
+ * the solve (p freshly rewritten by the previous iteration, as in gf_pcg),
+ * ms_iteration the average wall time of one iteration including the event
+ * records.  GF_ERR_BREAKDOWN when the iteration stops early.  No reference
+ * counterpart; the same per-kernel event timing as gf_profile_spmv. */
+int gf_profile_pcg(gf_ctx* ctx, gf_ell* m, const double* b, int32_t iters, double* ms_spmv,
+                   double* ms_iteration);
 /* Bandwidth probe: nrep launches of a plain streaming copy kernel of `bytes`
  * (multiple of 16; 16-byte aligned device pointers) on the context's stream;
  * *ms_avg = average launch time.  Measurement aid with no reference
