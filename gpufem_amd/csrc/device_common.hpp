@@ -136,6 +136,33 @@ __device__ inline T row_dot_node(int64_t i, int64_t n, const T* __restrict__ val
   return acc;
 }
 
+// 16-byte (FP64) or 8-byte (FP32) pair of block values, one per lane
+template <class T>
+struct PairOf;
+template <>
+struct PairOf<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct PairOf<float> {
+  typedef float type __attribute__((ext_vector_type(2)));
+};
+
+// the d*d values of block slot b for this lane (layout of bsell_vidx)
+template <int D, class T>
+__device__ inline void load_block(const T* __restrict__ bval, int64_t b, int lane, T* v) {
+  constexpr int DD = D * D;
+  using P2 = typename PairOf<T>::type;
+  const T* base = bval + b * DD * 64;
+#pragma unroll
+  for (int kp = 0; kp < DD / 2; ++kp) {
+    const P2 pr = __builtin_nontemporal_load(reinterpret_cast<const P2*>(base) + kp * 64 + lane);
+    v[2 * kp] = pr.x;
+    v[2 * kp + 1] = pr.y;
+  }
+  if constexpr (DD & 1) v[DD - 1] = __builtin_nontemporal_load(base + (DD - 1) * 64 + lane);
+}
+
 template <class T, int D>
 struct VecOf;
 template <>
@@ -180,9 +207,7 @@ __device__ inline void bsell_row(const BsellView& bs, const T* __restrict__ bval
     for (int u = 0; u < U; ++u) {
       const int64_t b = b0 + j + u;
       const int64_t c = int64_t(__builtin_nontemporal_load(bcol + b * 64 + lane)) * D;
-#pragma unroll
-      for (int k = 0; k < D * D; ++k)
-        v[u][k] = __builtin_nontemporal_load(bval + (b * (D * D) + k) * 64 + lane);
+      load_block<D, T>(bval, b, lane, v[u]);
       load_node<D, T>(x, c, xv[u]);
     }
 #pragma unroll
@@ -195,13 +220,13 @@ __device__ inline void bsell_row(const BsellView& bs, const T* __restrict__ bval
   for (; j < cnt; ++j) {
     const int64_t b = b0 + j;
     const int64_t c = int64_t(__builtin_nontemporal_load(bcol + b * 64 + lane)) * D;
-    T xv[D];
+    T xv[D], v[D * D];
+    load_block<D, T>(bval, b, lane, v);
     load_node<D, T>(x, c, xv);
 #pragma unroll
     for (int r = 0; r < D; ++r)
 #pragma unroll
-      for (int e = 0; e < D; ++e)
-        acc[r] += __builtin_nontemporal_load(bval + (b * (D * D) + r * D + e) * 64 + lane) * xv[e];
+      for (int e = 0; e < D; ++e) acc[r] += v[r * D + e] * xv[e];
   }
 }
 

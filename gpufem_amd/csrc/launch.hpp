@@ -102,10 +102,17 @@ hipError_t launch_build_coln(hipStream_t s, int64_t n, int32_t width, int d, con
 // matrix.  Node rows are sorted by length (descending, stable) inside windows
 // of kBsellWindow node rows and dealt into slices of 64 positions (one
 // wavefront); slice s has sptr[s+1] - sptr[s] block slots, its longest row.
-// Block slot b, lane l: node column bcol[b*64 + l]; block values
-// bval[((b*d + r)*d + e)*64 + l] = K(d R + r, d c + e) for the lane's node
-// row R = perm[s*64 + l] (-1: empty position).  Slots are in ascending column
-// order, so every output row sums its entries in the order of the ELL slots.
+// Block slot b, lane l: node column bcol[b*64 + l]; block value k = r d + e,
+// K(d R + r, d c + e) for the lane's node row R = perm[s*64 + l] (-1: empty
+// position), at bval[bsell_vidx(d*d, b, k, l)]: the d*d*64 values of a slot
+// hold the pairs (k, k+1), k even, lane-interleaved (one 16-byte load per
+// lane and pair), then the odd last value of 3 x 3 blocks lane by lane.
+// Slots are in ascending column order, so every output row sums its entries
+// in the order of the ELL slots.
+__host__ __device__ inline int64_t bsell_vidx(int dd, int64_t b, int k, int l) {
+  const int64_t base = b * dd * 64;
+  return k < (dd & ~1) ? base + ((k >> 1) * 64 + l) * 2 + (k & 1) : base + (dd - 1) * 64 + l;
+}
 struct BsellView {
   const int32_t* perm;
   const int64_t* sptr;

@@ -103,7 +103,7 @@ __global__ void k_bsell_vals(BsellView bs, int d, int64_t n, const double* __res
     for (int64_t j = 0; j < w; ++j)
       for (int r = 0; r < d; ++r)
         for (int e = 0; e < d; ++e)
-          bval[(((b0 + j) * d + r) * d + e) * 64 + lane] =
+          bval[bsell_vidx(d * d, b0 + j, r * d + e, lane)] =
               j < cnt ? val[int64_t(d) * R + r + (j * d + e) * n] : 0.0;
   }
 }

@@ -330,7 +330,10 @@ def test_node_block_spmv_is_bitwise_thread_per_row(gpu_ctx, et, dims, node_varia
                                              (gf.GF_Q8, (9, 5), 9), (gf.GF_Q4, (70, 33), 9),
                                              (gf.GF_Q8, (130, 70), 9),
                                              (gf.GF_BRICK20, (4, 3, 3), 10),
-                                             (gf.GF_BRICK8, (17, 16, 15), 10)])
+                                             (gf.GF_BRICK8, (17, 16, 15), 10),
+                                             (gf.GF_Q4, (1, 1), 9), (gf.GF_Q8, (1, 1), 7),
+                                             (gf.GF_BRICK8, (1, 1, 1), 10),
+                                             (gf.GF_BRICK20, (1, 1, 1), 8)])
 def test_block_sliced_spmv_is_bitwise_thread_per_row(gpu_ctx, et, dims, variant):
     """Variants 7/8 sweep the slices with the PCG grid, 9/10 give each group
     of four slices its own workgroup; all add a row's entries in slot order."""
