@@ -136,31 +136,34 @@ __device__ inline T row_dot_node(int64_t i, int64_t n, const T* __restrict__ val
   return acc;
 }
 
-// 16-byte (FP64) or 8-byte (FP32) pair of block values, one per lane
-template <class T>
-struct PairOf;
-template <>
-struct PairOf<double> {
-  typedef double type __attribute__((ext_vector_type(2)));
-};
-template <>
-struct PairOf<float> {
-  typedef float type __attribute__((ext_vector_type(2)));
-};
-
-// the d*d values of block slot b for this lane (layout of bsell_vidx)
+// the d*d values of block slot b for this lane, 16 bytes per load: FP64
+// pairs (bsell_vidx) or FP32 quads (bsell_vidx32)
 template <int D, class T>
 __device__ inline void load_block(const T* __restrict__ bval, int64_t b, int lane, T* v) {
   constexpr int DD = D * D;
-  using P2 = typename PairOf<T>::type;
   const T* base = bval + b * DD * 64;
+  if constexpr (sizeof(T) == 8) {
+    typedef double P2 __attribute__((ext_vector_type(2)));
 #pragma unroll
-  for (int kp = 0; kp < DD / 2; ++kp) {
-    const P2 pr = __builtin_nontemporal_load(reinterpret_cast<const P2*>(base) + kp * 64 + lane);
-    v[2 * kp] = pr.x;
-    v[2 * kp + 1] = pr.y;
+    for (int kp = 0; kp < DD / 2; ++kp) {
+      const P2 pr = __builtin_nontemporal_load(reinterpret_cast<const P2*>(base) + kp * 64 + lane);
+      v[2 * kp] = pr.x;
+      v[2 * kp + 1] = pr.y;
+    }
+    if constexpr (DD & 1) v[DD - 1] = __builtin_nontemporal_load(base + (DD - 1) * 64 + lane);
+  } else {
+    typedef float P4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+    for (int kq = 0; kq < DD / 4; ++kq) {
+      const P4 q = __builtin_nontemporal_load(reinterpret_cast<const P4*>(base) + kq * 64 + lane);
+      v[4 * kq] = q.x;
+      v[4 * kq + 1] = q.y;
+      v[4 * kq + 2] = q.z;
+      v[4 * kq + 3] = q.w;
+    }
+#pragma unroll
+    for (int k = DD & ~3; k < DD; ++k) v[k] = __builtin_nontemporal_load(base + k * 64 + lane);
   }
-  if constexpr (DD & 1) v[DD - 1] = __builtin_nontemporal_load(base + (DD - 1) * 64 + lane);
 }
 
 template <class T, int D>

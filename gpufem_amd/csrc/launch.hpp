@@ -113,6 +113,19 @@ __host__ __device__ inline int64_t bsell_vidx(int dd, int64_t b, int k, int l) {
   const int64_t base = b * dd * 64;
   return k < (dd & ~1) ? base + ((k >> 1) * 64 + l) * 2 + (k & 1) : base + (dd - 1) * 64 + l;
 }
+// The FP32 copy of the block values: quads (k .. k+3, k a multiple of 4)
+// lane-interleaved (one 16-byte load per lane and quad), then the ninth value
+// of 3 x 3 blocks lane by lane.
+__host__ __device__ inline int64_t bsell_vidx32(int dd, int64_t b, int k, int l) {
+  const int64_t base = b * dd * 64;
+  return k < (dd & ~3) ? base + ((k >> 2) * 64 + l) * 4 + (k & 3) : base + int64_t(k) * 64 + l;
+}
+// the layout of block values of type T (double: pairs, float: quads)
+template <class T>
+__host__ __device__ inline int64_t bsell_index(int dd, int64_t b, int k, int l) {
+  if constexpr (sizeof(T) == 4) return bsell_vidx32(dd, b, k, l);
+  else return bsell_vidx(dd, b, k, l);
+}
 struct BsellView {
   const int32_t* perm;
   const int64_t* sptr;

@@ -56,7 +56,8 @@ int ensure_val32(gf_ctx* ctx, gf_ell* m, int variant, const float** out) {
     const size_t cnt = size_t(m->bslots) * 64 * m->dim * m->dim;
     if (!m->d_bval32) GF_HIP_CHECK(ctx, hipMalloc(&m->d_bval32, cnt * sizeof(float)));
     if (m->bval32_version != m->val_version) {
-      GF_HIP_CHECK(ctx, launch_to_f32(ctx->stream, int64_t(cnt), m->d_bval, m->d_bval32));
+      GF_HIP_CHECK(ctx, launch_bsell_to_f32(ctx->stream, m->dim * m->dim, m->bslots, m->d_bval,
+                                            m->d_bval32));
       m->bval32_version = m->val_version;
     }
     *out = m->d_bval32;

@@ -41,6 +41,17 @@ __global__ void k_to_f32(int64_t count, const double* __restrict__ in, float* __
     out[i] = float(in[i]);
 }
 
+// FP32 copy of the block-sliced values, from the FP64 pairs to FP32 quads
+__global__ void k_bsell_to_f32(int dd, int64_t slots, const double* __restrict__ in,
+                               float* __restrict__ out) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < slots * 64;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t b = t >> 6;
+    const int l = int(t & 63);
+    for (int k = 0; k < dd; ++k) out[bsell_vidx32(dd, b, k, l)] = float(in[bsell_vidx(dd, b, k, l)]);
+  }
+}
+
 __global__ __launch_bounds__(kNT) void k_m_init(MixedBuffers B, int use_x0) {
   __shared__ double sh[kNT / 64];
   double rz = 0, rr = 0, bb = 0;
@@ -225,6 +236,13 @@ size_t mixed_state_bytes() { return sizeof(MixedState); }
 hipError_t launch_to_f32(hipStream_t s, int64_t count, const double* in, float* out) {
   hipLaunchKernelGGL(k_to_f32, dim3(grid_for(count, 256, 16384)), dim3(256), 0, s, count, in,
                      out);
+  return hipGetLastError();
+}
+
+hipError_t launch_bsell_to_f32(hipStream_t s, int dd, int64_t slots, const double* in,
+                              float* out) {
+  hipLaunchKernelGGL(k_bsell_to_f32, dim3(grid_for(slots * 64, 256, 16384)), dim3(256), 0, s, dd,
+                     slots, in, out);
   return hipGetLastError();
 }
 

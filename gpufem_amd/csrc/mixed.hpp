@@ -38,6 +38,10 @@ struct MixedBuffers {
 
 size_t mixed_state_bytes();
 hipError_t launch_to_f32(hipStream_t s, int64_t count, const double* in, float* out);
+// FP32 copy of block-sliced values (slots block slots of dd values, FP64
+// pair layout in, FP32 quad layout out)
+hipError_t launch_bsell_to_f32(hipStream_t s, int dd, int64_t slots, const double* in,
+                              float* out);
 hipError_t launch_mixed_setup(hipStream_t s, const MixedBuffers& B, int use_x0, double rtol,
                               double delta, int maxit);
 hipError_t launch_mixed_iteration(hipStream_t s, const MixedBuffers& B, int parity);

@@ -223,7 +223,7 @@ __device__ inline void for_chunk_node_rows(const SpmmArgs& A, const VT* __restri
         const int64_t b = b0 + jb + u;
         const int64_t c = int64_t(A.col[b * 64 + lp]) * D;
 #pragma unroll
-        for (int k = 0; k < D * D; ++k) v[u][k] = T(val[bsell_vidx(D * D, b, k, lp)]);
+        for (int k = 0; k < D * D; ++k) v[u][k] = T(val[bsell_index<VT>(D * D, b, k, lp)]);
 #pragma unroll
         for (int e = 0; e < D; ++e) load_chunk(xin + (c + e) * S + c0, xv[u][e]);
       }
@@ -241,7 +241,7 @@ __device__ inline void for_chunk_node_rows(const SpmmArgs& A, const VT* __restri
       const int64_t c = int64_t(A.col[b * 64 + lp]) * D;
       T v[D * D], xv[D][kC];
 #pragma unroll
-      for (int k = 0; k < D * D; ++k) v[k] = T(val[bsell_vidx(D * D, b, k, lp)]);
+      for (int k = 0; k < D * D; ++k) v[k] = T(val[bsell_index<VT>(D * D, b, k, lp)]);
 #pragma unroll
       for (int e = 0; e < D; ++e) load_chunk(xin + (c + e) * S + c0, xv[e]);
 #pragma unroll
