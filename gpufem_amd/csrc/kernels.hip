@@ -26,25 +26,27 @@ template <int DIM>
 __device__ inline double det_inv(const double (&J)[DIM][DIM], double (&Ji)[DIM][DIM]) {
   if constexpr (DIM == 2) {
     double d = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-    Ji[0][0] = J[1][1] / d;
-    Ji[0][1] = -J[0][1] / d;
-    Ji[1][0] = -J[1][0] / d;
-    Ji[1][1] = J[0][0] / d;
+    const double id = 1.0 / d;  // one division, four products
+    Ji[0][0] = J[1][1] * id;
+    Ji[0][1] = -J[0][1] * id;
+    Ji[1][0] = -J[1][0] * id;
+    Ji[1][1] = J[0][0] * id;
     return d;
   } else {
     double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
     double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
     double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
     double d = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-    Ji[0][0] = c00 / d;
-    Ji[1][0] = c01 / d;
-    Ji[2][0] = c02 / d;
-    Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / d;
-    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / d;
-    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / d;
-    Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / d;
-    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / d;
-    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / d;
+    const double id = 1.0 / d;  // one division, nine products
+    Ji[0][0] = c00 * id;
+    Ji[1][0] = c01 * id;
+    Ji[2][0] = c02 * id;
+    Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
     return d;
   }
 }
