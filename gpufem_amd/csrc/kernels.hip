@@ -501,35 +501,64 @@ __global__ __launch_bounds__(64) void k_assemble_gather(GArgs ga) {
           for (int k = 0; k < DIM; ++k) acc += Ji[i][k] * dN[k * NEN + la];
           gA[i] = acc;
         }
+        if constexpr (DIM == 2 && RPT == 2) {
+          // both rows of the node: accumulate the four products
+          // T_pq = sum_g (sc gA_p)(gB_q) per node b; the plane-stress
+          // combinations are formed once per element below
+          const double sA[2] = {sc * gA[0], sc * gA[1]};
 #pragma unroll
-        for (int b = 0; b < NEN; ++b) {
-          double gB[DIM];
+          for (int b = 0; b < NEN; ++b) {
+            double gB[2];
 #pragma unroll
-          for (int i = 0; i < DIM; ++i) {
-            double acc = 0.0;
+            for (int i = 0; i < 2; ++i)
+              gB[i] = Ji[i][0] * dN[b] + Ji[i][1] * dN[NEN + b];
 #pragma unroll
-            for (int k = 0; k < DIM; ++k) acc += Ji[i][k] * dN[k * NEN + b];
-            gB[i] = acc;
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+              for (int q = 0; q < 2; ++q) kl[b][p][q] += sA[p] * gB[q];
           }
-          double dot = 0.0;
+        } else {
 #pragma unroll
-          for (int i = 0; i < DIM; ++i) dot += gA[i] * gB[i];
+          for (int b = 0; b < NEN; ++b) {
+            double gB[DIM];
 #pragma unroll
-          for (int r = 0; r < RPT; ++r) {
-            const int i = r0 + r;
+            for (int i = 0; i < DIM; ++i) {
+              double acc = 0.0;
 #pragma unroll
-            for (int j = 0; j < DIM; ++j) {
-              double kij;
-              if constexpr (DIM == 2) {
-                if (i == j) kij = a.c0 * gA[i] * gB[i] + a.c2 * gA[1 - i] * gB[1 - i];
-                else kij = a.c1 * gA[i] * gB[j] + a.c2 * gA[j] * gB[i];
-              } else {
-                if (i == j) kij = a.c0 * gA[i] * gB[i] + a.c2 * (dot - gA[i] * gB[i]);
-                else kij = a.c1 * gA[i] * gB[j] + a.c2 * gA[j] * gB[i];
+              for (int k = 0; k < DIM; ++k) acc += Ji[i][k] * dN[k * NEN + b];
+              gB[i] = acc;
+            }
+            double dot = 0.0;
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) dot += gA[i] * gB[i];
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+              const int i = r0 + r;
+#pragma unroll
+              for (int j = 0; j < DIM; ++j) {
+                double kij;
+                if constexpr (DIM == 2) {
+                  if (i == j) kij = a.c0 * gA[i] * gB[i] + a.c2 * gA[1 - i] * gB[1 - i];
+                  else kij = a.c1 * gA[i] * gB[j] + a.c2 * gA[j] * gB[i];
+                } else {
+                  if (i == j) kij = a.c0 * gA[i] * gB[i] + a.c2 * (dot - gA[i] * gB[i]);
+                  else kij = a.c1 * gA[i] * gB[j] + a.c2 * gA[j] * gB[i];
+                }
+                kl[b][r][j] += sc * kij;
               }
-              kl[b][r][j] += sc * kij;
             }
           }
+        }
+      }
+      if constexpr (DIM == 2 && RPT == 2) {
+        // K_ii = c0 T_ii + c2 T_kk (k the other direction), K_ij = c1 T_ij + c2 T_ji
+#pragma unroll
+        for (int b = 0; b < NEN; ++b) {
+          const double t00 = kl[b][0][0], t01 = kl[b][0][1], t10 = kl[b][1][0], t11 = kl[b][1][1];
+          kl[b][0][0] = a.c0 * t00 + a.c2 * t11;
+          kl[b][1][1] = a.c0 * t11 + a.c2 * t00;
+          kl[b][0][1] = a.c1 * t01 + a.c2 * t10;
+          kl[b][1][0] = a.c1 * t10 + a.c2 * t01;
         }
       }
       const uint8_t* sl = a.slot + (e * NEN + la) * NEN;
