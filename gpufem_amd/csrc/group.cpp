@@ -67,6 +67,17 @@ void group_release(gf_group* g) {
   g->mesh = nullptr;
 }
 
+// The grouped SpMV runs thread per row over the ELL arrays or over the node
+// columns (kNode2/3); when the tuner picked a block-sliced variant for the
+// merged matrix, the node-column variant of the same dimension stands in
+// (the per-mesh tiles of the grouped kernels are row ranges).
+int group_variant(const gf_ell* m) {
+  const int v = m->plan ? m->plan->variant : -1;
+  if (v >= 0 && v <= kThreadU8) return v;
+  if ((is_node_variant(v) || is_bsell_variant(v)) && node_variant(m) >= 0) return node_variant(m);
+  return kThreadU8;
+}
+
 GroupBuffers buffers(gf_group* g) {
   GroupBuffers G;
   gf_ell* m = g->ell;
@@ -74,9 +85,7 @@ GroupBuffers buffers(gf_group* g) {
   G.nmesh = g->nmesh;
   G.ntiles = g->ntiles;
   G.nblocks = group_blocks(g->ntiles);
-  G.variant = (m->plan && m->plan->variant >= 0 && m->plan->variant <= kThreadU8)
-                  ? m->plan->variant
-                  : kThreadU8;
+  G.variant = group_variant(m);
   G.off = g->d_off;
   G.tile_ptr = g->d_tile_ptr;
   G.tile_mesh = g->d_tile_mesh;
@@ -84,6 +93,7 @@ GroupBuffers buffers(gf_group* g) {
   G.tile_r1 = g->d_tile_r1;
   G.val = m->d_val;
   G.col = m->d_col;
+  G.scol = is_node_variant(G.variant) ? m->d_coln : m->d_col;
   G.rownnz = m->d_rownnz;
   G.diag = m->d_diag;
   G.dinv = m->d_dinv;
@@ -296,6 +306,7 @@ int gf_group_pcg(gf_ctx* ctx, gf_group* g, const double* b, double* x, const gf_
   if (st) return st;
   if (m->plan->variant < 0 && (st = tune_spmv(ctx, m, -1, nullptr, nullptr))) return st;
   if ((st = prepare_jacobi(ctx, m))) return st;
+  if ((st = prepare_variant(ctx, m, group_variant(m)))) return st;
   const int64_t n = m->n;
   const size_t vb = size_t(n) * sizeof(double);
   const hipMemcpyKind in = opts.device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;

@@ -126,6 +126,7 @@ __global__ __launch_bounds__(kNT) void k_g_spmv_pq(GroupBuffers G) {
   if (G.st->done) return;
   __shared__ double sh[kNT / 64];
   constexpr int U = V == kThreadU1 ? 1 : (V == kThreadU4 ? 4 : 8);
+  constexpr int D = V == kNode2 ? 2 : 3;
   tiles_loop(G, [&](int64_t t, int m) {
     if (G.done[m]) {  // uniform across the workgroup
       if (threadIdx.x == 0) G.part[t] = 0.0;
@@ -134,7 +135,11 @@ __global__ __launch_bounds__(kNT) void k_g_spmv_pq(GroupBuffers G) {
     const int64_t i = G.tile_r0[t] + threadIdx.x;
     double pq = 0.0;
     if (i < G.tile_r1[t]) {
-      const double qi = row_dot<U, double>(i, G.n, G.val, G.col, G.rownnz[i], G.p);
+      double qi;
+      if constexpr (V == kNode2 || V == kNode3)
+        qi = row_dot_node<D, double>(i, G.n, G.val, G.scol, G.rownnz[i], G.p);
+      else
+        qi = row_dot<U, double>(i, G.n, G.val, G.scol, G.rownnz[i], G.p);
       G.q[i] = qi;
       pq = G.p[i] * qi;
     }
@@ -314,6 +319,8 @@ hipError_t launch_group_iteration(hipStream_t s, const GroupBuffers& G) {
   switch (G.variant) {
     case kThreadU1: hipLaunchKernelGGL(k_g_spmv_pq<kThreadU1>, g, t, 0, s, G); break;
     case kThreadU4: hipLaunchKernelGGL(k_g_spmv_pq<kThreadU4>, g, t, 0, s, G); break;
+    case kNode2: hipLaunchKernelGGL(k_g_spmv_pq<kNode2>, g, t, 0, s, G); break;
+    case kNode3: hipLaunchKernelGGL(k_g_spmv_pq<kNode3>, g, t, 0, s, G); break;
     default: hipLaunchKernelGGL(k_g_spmv_pq<kThreadU8>, g, t, 0, s, G); break;
   }
   if ((e = hipGetLastError()) != hipSuccess) return e;

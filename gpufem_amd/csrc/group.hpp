@@ -24,7 +24,8 @@ struct GroupBuffers {
   const int64_t* tile_r0;    // [ntiles]
   const int64_t* tile_r1;    // [ntiles]
   double* val;
-  const int32_t* col;
+  const int32_t* col;   // ELL columns
+  const int32_t* scol;  // columns of the iteration's SpMV (node columns for kNode2/3)
   const int32_t* rownnz;
   const int32_t* diag;
   const double* dinv;
