@@ -222,6 +222,7 @@ static void ell_release(gf_ell* m) {
   dfree(m->d_bperm);
   dfree(m->d_bsptr);
   dfree(m->d_bcol);
+  dfree(m->d_bcoff);
   dfree(m->d_bval);
   dfree(m->d_bval32);
   dfree(m->d_rownnz);

@@ -191,15 +191,26 @@ __device__ inline void load_node(const T* __restrict__ x, int64_t c, T* xv) {
 }
 
 // Node row R of the block-sliced copy, lane l of slice s: acc[r] = (K x)_{dR+r}
-// over cnt block slots from b0 (the row's blocks, then zero padding).
+// over cnt block slots from b0 (the row's blocks, then zero padding).  The
+// node column of a slot is base + coff[...] (16-bit offsets, OFF) or bcol[...].
 // Entries are added in ascending column order, as row_dot adds its slots.
 // U blocks per step (measured on the Q8 plate and the Brick8 cube: 4 for
 // 2 x 2 blocks, 3 for 3 x 3; fewer lose more in-flight loads than the extra
 // occupancy gives back, more cost occupancy).
-template <int D, class T>
+template <int D, class T, bool OFF>
+__device__ inline int64_t bsell_col(const int32_t* __restrict__ bcol,
+                                    const int16_t* __restrict__ coff, int32_t base, int64_t b,
+                                    int lane) {
+  if constexpr (OFF)
+    return int64_t(base + int32_t(__builtin_nontemporal_load(coff + b * 64 + lane))) * D;
+  else
+    return int64_t(__builtin_nontemporal_load(bcol + b * 64 + lane)) * D;
+}
+
+template <int D, class T, bool OFF>
 __device__ inline void bsell_row(const BsellView& bs, const T* __restrict__ bval,
                                  const int32_t* __restrict__ bcol, int64_t b0, int32_t cnt,
-                                 int lane, const T* __restrict__ x, T* acc) {
+                                 int lane, const T* __restrict__ x, T* acc, int32_t base) {
   constexpr int U = D == 2 ? 4 : 3;
 #pragma unroll
   for (int r = 0; r < D; ++r) acc[r] = T(0);
@@ -209,7 +220,7 @@ __device__ inline void bsell_row(const BsellView& bs, const T* __restrict__ bval
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int64_t b = b0 + j + u;
-      const int64_t c = int64_t(__builtin_nontemporal_load(bcol + b * 64 + lane)) * D;
+      const int64_t c = bsell_col<D, T, OFF>(bcol, bs.coff, base, b, lane);
       load_block<D, T>(bval, b, lane, v[u]);
       load_node<D, T>(x, c, xv[u]);
     }
@@ -222,7 +233,7 @@ __device__ inline void bsell_row(const BsellView& bs, const T* __restrict__ bval
   }
   for (; j < cnt; ++j) {
     const int64_t b = b0 + j;
-    const int64_t c = int64_t(__builtin_nontemporal_load(bcol + b * 64 + lane)) * D;
+    const int64_t c = bsell_col<D, T, OFF>(bcol, bs.coff, base, b, lane);
     T xv[D], v[D * D];
     load_block<D, T>(bval, b, lane, v);
     load_node<D, T>(x, c, xv);
@@ -255,9 +266,10 @@ __device__ inline void for_rows_spmv(int lb, int nlb, int64_t n, const T* __rest
       // sums, up to the sign of a zero sum)
       const int64_t b0 = bs.sptr[sl];
       const int32_t w = int32_t(bs.sptr[sl + 1] - b0);
-      T acc[D];
-      bsell_row<D, T>(bs, val, col, b0, w, lane, x, acc);
       const int32_t R = bs.perm[sl * 64 + lane];
+      T acc[D];
+      if (bs.coff) bsell_row<D, T, true>(bs, val, col, b0, w, lane, x, acc, R >= 0 ? R : 0);
+      else bsell_row<D, T, false>(bs, val, col, b0, w, lane, x, acc, 0);
       if (R < 0) continue;
 #pragma unroll
       for (int r = 0; r < D; ++r) f(int64_t(D) * R + r, acc[r]);

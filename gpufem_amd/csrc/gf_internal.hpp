@@ -81,6 +81,7 @@ struct gf_ell {
   int64_t* d_bsptr = nullptr;
   int64_t bslices = 0, bslots = 0;
   int32_t* d_bcol = nullptr;
+  int16_t* d_bcoff = nullptr;  // 16-bit node-column offsets (null: some did not fit)
   double* d_bval = nullptr;
   float* d_bval32 = nullptr;
   // values change -> val_version++; the copies remember the version they hold

@@ -130,10 +130,16 @@ struct BsellView {
   const int32_t* perm;
   const int64_t* sptr;
   int64_t nslices;
+  // node column - node row per block slot and lane (padding 0), when every
+  // offset of the matrix fits 16 bits; null: the single-RHS kernels read bcol
+  const int16_t* coff;
 };
 constexpr int kBsellWindow = 2048;
 hipError_t launch_bsell_cols(hipStream_t s, const BsellView& bs, int d, int64_t n,
                              const int32_t* col, const int32_t* rownnz, int32_t* bcol);
+// coff[b*64 + l] = bcol[b*64 + l] - max(R, 0); *overflow = 1 when one does not fit
+hipError_t launch_bsell_offsets(hipStream_t s, const BsellView& bs, const int32_t* bcol,
+                                int16_t* coff, int* overflow);
 hipError_t launch_bsell_vals(hipStream_t s, const BsellView& bs, int d, int64_t n,
                              const double* val, const int32_t* rownnz, double* bval);
 constexpr int kMaxPcgBlocks = 2048;

@@ -133,7 +133,9 @@ int ensure_coln(gf_ctx* ctx, gf_ell* m) {
   return GF_OK;
 }
 
-BsellView bsell_view(const gf_ell* m) { return BsellView{m->d_bperm, m->d_bsptr, m->bslices}; }
+BsellView bsell_view(const gf_ell* m) {
+  return BsellView{m->d_bperm, m->d_bsptr, m->bslices, m->d_bcoff};
+}
 
 // Structure of the block-sliced copy: node rows sorted by length inside
 // windows, slices of 64 positions, one block slot per entry of the longest
@@ -170,6 +172,22 @@ int ensure_bsell(gf_ctx* ctx, gf_ell* m) {
   GF_HIP_CHECK(ctx, hipMemcpy(m->d_bsptr, sptr.data(), sptr.size() * 8, hipMemcpyHostToDevice));
   GF_HIP_CHECK(ctx, launch_bsell_cols(ctx->stream, bsell_view(m), d, m->n, m->d_col, m->d_rownnz,
                                       m->d_bcol));
+  // 16-bit node-column offsets for the single-RHS kernels when they all fit
+  GF_HIP_CHECK(ctx, hipMalloc(&m->d_bcoff, size_t(m->bslots) * 64 * sizeof(int16_t)));
+  int* d_over = reinterpret_cast<int*>(static_cast<char*>(ctx->d_state) + 160);
+  GF_HIP_CHECK(ctx, hipMemsetAsync(d_over, 0, sizeof(int), ctx->stream));
+  {
+    BsellView v = bsell_view(m);
+    v.coff = nullptr;
+    GF_HIP_CHECK(ctx, launch_bsell_offsets(ctx->stream, v, m->d_bcol, m->d_bcoff, d_over));
+  }
+  int over = 0;
+  GF_HIP_CHECK(ctx, hipMemcpyAsync(&over, d_over, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  GF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (over) {
+    GF_HIP_CHECK(ctx, hipFree(m->d_bcoff));
+    m->d_bcoff = nullptr;
+  }
   m->bval_version = 0;
   m->bval32_version = 0;
   return GF_OK;
@@ -197,7 +215,7 @@ SpmvOperands spmv_operands(const gf_ell* m, int variant) {
   SpmvOperands o;
   o.val = m->d_val;
   o.col = m->d_col;
-  o.bs = BsellView{nullptr, nullptr, 0};
+  o.bs = BsellView{nullptr, nullptr, 0, nullptr};
   if (is_node_variant(variant)) {
     o.col = m->d_coln;
   } else if (is_bsell_variant(variant)) {

@@ -91,6 +91,25 @@ __global__ void k_bsell_cols(BsellView bs, int d, int64_t n, const int32_t* __re
   }
 }
 
+__global__ void k_bsell_offsets(BsellView bs, const int32_t* __restrict__ bcol,
+                                int16_t* __restrict__ coff, int* overflow) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < bs.nslices * 64;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t sl = t >> 6;
+    const int lane = int(t & 63);
+    const int32_t R = bs.perm[t];
+    const int32_t base = R >= 0 ? R : 0;
+    const int64_t b0 = bs.sptr[sl], w = bs.sptr[sl + 1] - b0;
+    bool bad = false;
+    for (int64_t j = 0; j < w; ++j) {
+      const int32_t d = bcol[(b0 + j) * 64 + lane] - base;
+      bad |= d < -32768 || d > 32767;
+      coff[(b0 + j) * 64 + lane] = int16_t(d);
+    }
+    if (bad) atomicOr(overflow, 1);
+  }
+}
+
 __global__ void k_bsell_vals(BsellView bs, int d, int64_t n, const double* __restrict__ val,
                              const int32_t* __restrict__ rownnz, double* __restrict__ bval) {
   for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < bs.nslices * 64;
@@ -457,6 +476,13 @@ hipError_t launch_bsell_cols(hipStream_t s, const BsellView& bs, int d, int64_t 
   return hipGetLastError();
 }
 
+hipError_t launch_bsell_offsets(hipStream_t s, const BsellView& bs, const int32_t* bcol,
+                                int16_t* coff, int* overflow) {
+  hipLaunchKernelGGL(k_bsell_offsets, dim3(grid_for(bs.nslices * 64, kNT, 8192)), dim3(kNT), 0, s,
+                     bs, bcol, coff, overflow);
+  return hipGetLastError();
+}
+
 hipError_t launch_bsell_vals(hipStream_t s, const BsellView& bs, int d, int64_t n,
                              const double* val, const int32_t* rownnz, double* bval) {
   hipLaunchKernelGGL(k_bsell_vals, dim3(grid_for(bs.nslices * 64, kNT, 8192)), dim3(kNT), 0, s, bs,
@@ -476,7 +502,7 @@ int spmv_grid(int variant, const BsellView& bs, int nblocks) {
 hipError_t launch_spmv(hipStream_t s, int variant, int64_t n, const double* val,
                        const int32_t* col, const int32_t* rownnz, const double* x, double* y,
                        int nrhs, const BsellView* bs) {
-  const BsellView view = bs ? *bs : BsellView{nullptr, nullptr, 0};
+  const BsellView view = bs ? *bs : BsellView{nullptr, nullptr, 0, nullptr};
   if (is_bsell_variant(variant) && !view.perm) return hipErrorInvalidValue;
   const dim3 g(spmv_grid(variant, view, spmv_blocks(n)));
   for (int k = 0; k < nrhs; ++k) {
