@@ -80,6 +80,9 @@ struct gf_ell {
   int32_t* d_bperm = nullptr;
   int64_t* d_bsptr = nullptr;
   int64_t bslices = 0, bslots = 0;
+  // node rows where block-sliced windows restart (a grouped batch: the first
+  // node of every mesh, then the node count), and the first slice of each
+  std::vector<int64_t> bsell_breaks, bsell_break_slice;
   int32_t* d_bcol = nullptr;
   int16_t* d_bcoff = nullptr;  // 16-bit node-column offsets (null: some did not fit)
   double* d_bval = nullptr;

@@ -26,6 +26,11 @@ struct gf_group {
   int32_t* d_tile_mesh = nullptr;
   double *b = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
   double* part = nullptr;
+  // slice groups of the block-sliced SpMV (built on the first solve that uses it)
+  int64_t nsg = 0;
+  int64_t *d_sg_ptr = nullptr, *d_sg_s0 = nullptr, *d_sg_s1 = nullptr;
+  int32_t* d_sg_mesh = nullptr;
+  double* part_sg = nullptr;
   double* per_mesh = nullptr;   // rho, alpha, beta, rr, bb, pen, res: 7 * nmesh
   int32_t* per_mesh_i = nullptr;  // it, done, status: 3 * nmesh
   gf::GroupState* st = nullptr;
@@ -60,6 +65,11 @@ void group_release(gf_group* g) {
   double** v[] = {&g->b, &g->x, &g->r, &g->z, &g->p, &g->q, &g->part, &g->per_mesh};
   for (double** d : v) dfree(*d);
   dfree(g->per_mesh_i);
+  dfree(g->d_sg_ptr);
+  dfree(g->d_sg_s0);
+  dfree(g->d_sg_s1);
+  dfree(g->d_sg_mesh);
+  dfree(g->part_sg);
   dfree(g->st);
   if (g->ell) gf_ell_free(g->ell);
   if (g->mesh) gf_mesh_free(g->mesh);
@@ -68,14 +78,46 @@ void group_release(gf_group* g) {
 }
 
 // The grouped SpMV runs thread per row over the ELL arrays or over the node
-// columns (kNode2/3); when the tuner picked a block-sliced variant for the
-// merged matrix, the node-column variant of the same dimension stands in
-// (the per-mesh tiles of the grouped kernels are row ranges).
+// columns (kNode2/3), or over the block-sliced copy whose windows restart at
+// every mesh (a slice belongs to one mesh) when the tuner picked a
+// block-sliced variant for the merged matrix.
 int group_variant(const gf_ell* m) {
   const int v = m->plan ? m->plan->variant : -1;
   if (v >= 0 && v <= kThreadU8) return v;
-  if ((is_node_variant(v) || is_bsell_variant(v)) && node_variant(m) >= 0) return node_variant(m);
+  if (is_bsell_variant(v)) return v;
+  if (is_node_variant(v) && node_variant(m) >= 0) return node_variant(m);
   return kThreadU8;
+}
+
+// slice groups per mesh from the block-sliced structure (once)
+int ensure_slice_groups(gf_ctx* ctx, gf_group* g) {
+  if (g->d_sg_ptr) return GF_OK;
+  const std::vector<int64_t>& bs = g->ell->bsell_break_slice;
+  if (int(bs.size()) != g->nmesh + 1)
+    return set_error(ctx, GF_ERR_HIP, "gf_group_pcg: block-sliced windows do not follow the meshes");
+  std::vector<int64_t> ptr(size_t(g->nmesh) + 1, 0), s0, s1;
+  std::vector<int32_t> mesh;
+  for (int k = 0; k < g->nmesh; ++k) {
+    for (int64_t s = bs[size_t(k)]; s < bs[size_t(k) + 1]; s += 4) {
+      s0.push_back(s);
+      s1.push_back(std::min<int64_t>(s + 4, bs[size_t(k) + 1]));
+      mesh.push_back(k);
+    }
+    ptr[size_t(k) + 1] = int64_t(mesh.size());
+  }
+  g->nsg = int64_t(mesh.size());
+  const size_t ns = std::max<size_t>(mesh.size(), 1);
+  hipError_t e = hipMalloc(&g->d_sg_ptr, ptr.size() * 8);
+  if (e == hipSuccess) e = hipMalloc(&g->d_sg_s0, ns * 8);
+  if (e == hipSuccess) e = hipMalloc(&g->d_sg_s1, ns * 8);
+  if (e == hipSuccess) e = hipMalloc(&g->d_sg_mesh, ns * 4);
+  if (e == hipSuccess) e = hipMalloc(&g->part_sg, ns * 8);
+  if (e == hipSuccess) e = hipMemcpy(g->d_sg_ptr, ptr.data(), ptr.size() * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(g->d_sg_s0, s0.data(), s0.size() * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(g->d_sg_s1, s1.data(), s1.size() * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(g->d_sg_mesh, mesh.data(), mesh.size() * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return set_error(ctx, GF_ERR_NOMEM, std::string("gf_group_pcg: ") + hipGetErrorString(e));
+  return GF_OK;
 }
 
 GroupBuffers buffers(gf_group* g) {
@@ -94,6 +136,17 @@ GroupBuffers buffers(gf_group* g) {
   G.val = m->d_val;
   G.col = m->d_col;
   G.scol = is_node_variant(G.variant) ? m->d_coln : m->d_col;
+  const bool bsell = is_bsell_variant(G.variant) && g->d_sg_ptr;
+  G.bsell_d = bsell ? m->dim : 0;
+  G.bval = bsell ? m->d_bval : nullptr;
+  G.bcol = bsell ? m->d_bcol : nullptr;
+  G.bs = BsellView{m->d_bperm, m->d_bsptr, m->bslices, m->d_bcoff};
+  G.nsg = g->nsg;
+  G.sg_ptr = g->d_sg_ptr;
+  G.sg_mesh = g->d_sg_mesh;
+  G.sg_s0 = g->d_sg_s0;
+  G.sg_s1 = g->d_sg_s1;
+  G.part_sg = g->part_sg;
   G.rownnz = m->d_rownnz;
   G.diag = m->d_diag;
   G.dinv = m->d_dinv;
@@ -177,6 +230,9 @@ int gf_group_create(gf_ctx* ctx, int32_t nmesh, const gf_mesh_desc* meshes, gf_g
     group_release(g.get());
     return st;
   }
+  // block-sliced windows restart at every mesh, so that a slice belongs to one mesh
+  g->ell->bsell_breaks.clear();
+  for (int k = 0; k <= nmesh; ++k) g->ell->bsell_breaks.push_back(g->off[size_t(k)] / ei.dim);
   // tiles that never cross a mesh boundary
   std::vector<int32_t> tmesh;
   std::vector<int64_t> tr0, tr1;
@@ -307,6 +363,7 @@ int gf_group_pcg(gf_ctx* ctx, gf_group* g, const double* b, double* x, const gf_
   if (m->plan->variant < 0 && (st = tune_spmv(ctx, m, -1, nullptr, nullptr))) return st;
   if ((st = prepare_jacobi(ctx, m))) return st;
   if ((st = prepare_variant(ctx, m, group_variant(m)))) return st;
+  if (is_bsell_variant(group_variant(m)) && (st = ensure_slice_groups(ctx, g))) return st;
   const int64_t n = m->n;
   const size_t vb = size_t(n) * sizeof(double);
   const hipMemcpyKind in = opts.device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;

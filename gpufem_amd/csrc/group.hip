@@ -148,13 +148,54 @@ __global__ __launch_bounds__(kNT) void k_g_spmv_pq(GroupBuffers G) {
   });
 }
 
+// q = K p over the block-sliced copy: workgroup gi takes slice group gi (at
+// most four slices of one mesh, one per wave); partial p.q per group
+template <int D>
+__global__ __launch_bounds__(kNT) void k_g_spmv_pq_bs(GroupBuffers G) {
+  if (G.st->done) return;
+  __shared__ double sh[kNT / 64];
+  const int lane = threadIdx.x & 63;
+  for (int64_t gi = blockIdx.x; gi < G.nsg; gi += gridDim.x) {
+    const int m = G.sg_mesh[gi];
+    if (G.done[m]) {  // uniform across the workgroup
+      if (threadIdx.x == 0) G.part_sg[gi] = 0.0;
+      continue;
+    }
+    const int64_t sl = G.sg_s0[gi] + (threadIdx.x >> 6);
+    double pq = 0.0;
+    if (sl < G.sg_s1[gi]) {
+      const int64_t b0 = G.bs.sptr[sl];
+      const int32_t w = int32_t(G.bs.sptr[sl + 1] - b0);
+      const int32_t R = G.bs.perm[sl * 64 + lane];
+      double acc[D];
+      if (G.bs.coff)
+        bsell_row<D, double, true>(G.bs, G.bval, G.bcol, b0, w, lane, G.p, acc, R >= 0 ? R : 0);
+      else
+        bsell_row<D, double, false>(G.bs, G.bval, G.bcol, b0, w, lane, G.p, acc, 0);
+      if (R >= 0) {
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+          const int64_t i = int64_t(D) * R + r;
+          G.q[i] = acc[r];
+          pq += G.p[i] * acc[r];
+        }
+      }
+    }
+    pq = block_sum(pq, sh);
+    if (threadIdx.x == 0) G.part_sg[gi] = pq;
+  }
+}
+
 __global__ __launch_bounds__(kNT) void k_g_alpha(GroupBuffers G) {
   if (G.st->done) return;
   for (int m = threadIdx.x; m < G.nmesh; m += kNT) {
     double a = 0.0;
     if (!G.done[m]) {
       double pq = 0.0;
-      for (int64_t t = G.tile_ptr[m]; t < G.tile_ptr[m + 1]; ++t) pq += G.part[t];
+      if (G.bsell_d)
+        for (int64_t s = G.sg_ptr[m]; s < G.sg_ptr[m + 1]; ++s) pq += G.part_sg[s];
+      else
+        for (int64_t t = G.tile_ptr[m]; t < G.tile_ptr[m + 1]; ++t) pq += G.part[t];
       if (pq > 0.0) {
         a = G.rho[m] / pq;
       } else {
@@ -316,7 +357,10 @@ hipError_t launch_group_setup(hipStream_t s, const GroupBuffers& G, int use_x0, 
 hipError_t launch_group_iteration(hipStream_t s, const GroupBuffers& G) {
   hipError_t e;
   const dim3 g(G.nblocks), t(kNT);
-  switch (G.variant) {
+  const dim3 gs(G.nsg < 65536 ? (G.nsg > 0 ? G.nsg : 1) : 65536);
+  if (G.bsell_d == 2) hipLaunchKernelGGL(k_g_spmv_pq_bs<2>, gs, t, 0, s, G);
+  else if (G.bsell_d == 3) hipLaunchKernelGGL(k_g_spmv_pq_bs<3>, gs, t, 0, s, G);
+  else switch (G.variant) {
     case kThreadU1: hipLaunchKernelGGL(k_g_spmv_pq<kThreadU1>, g, t, 0, s, G); break;
     case kThreadU4: hipLaunchKernelGGL(k_g_spmv_pq<kThreadU4>, g, t, 0, s, G); break;
     case kNode2: hipLaunchKernelGGL(k_g_spmv_pq<kNode2>, g, t, 0, s, G); break;

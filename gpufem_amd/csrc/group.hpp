@@ -32,6 +32,18 @@ struct GroupBuffers {
   const double* b;
   double *x, *r, *z, *p, *q;
   double* part;  // [3 * ntiles]
+  // block-sliced SpMV (bsell_d = 2, 3; 0: thread per row over col/scol):
+  // slice groups of at most four slices of one mesh, one workgroup each
+  int bsell_d;
+  const double* bval;
+  const int32_t* bcol;
+  BsellView bs;
+  int64_t nsg;
+  const int64_t* sg_ptr;   // [nmesh+1] first slice group of each mesh
+  const int32_t* sg_mesh;  // [nsg]
+  const int64_t* sg_s0;    // [nsg] first slice
+  const int64_t* sg_s1;    // [nsg] end slice
+  double* part_sg;         // [nsg] p.q partials
   double *rho, *alpha, *beta, *rr, *bb, *pen;  // [nmesh]
   int32_t *it, *done, *status;                 // [nmesh]
   GroupState* st;

@@ -148,11 +148,24 @@ int ensure_bsell(gf_ctx* ctx, gf_ell* m) {
   GF_HIP_CHECK(ctx, hipMemcpy(rn.data(), m->d_rownnz, size_t(m->n) * 4, hipMemcpyDeviceToHost));
   std::vector<int32_t> cnt(static_cast<size_t>(N)), perm;
   for (int64_t R = 0; R < N; ++R) cnt[size_t(R)] = rn[size_t(d * R)] / d;
-  const int64_t nwin = (N + kBsellWindow - 1) / kBsellWindow;
+  // windows of kBsellWindow node rows, restarting at every break
+  std::vector<int64_t> brk = m->bsell_breaks;
+  if (brk.empty() || brk.front() != 0) brk.insert(brk.begin(), 0);
+  if (brk.back() != N) brk.push_back(N);
+  std::vector<std::pair<int64_t, int64_t>> wins;
+  std::vector<size_t> brk_win;  // first window of each segment
+  for (size_t k = 0; k + 1 < brk.size(); ++k) {
+    brk_win.push_back(wins.size());
+    for (int64_t r = brk[k]; r < brk[k + 1]; r += kBsellWindow)
+      wins.emplace_back(r, std::min<int64_t>(brk[k + 1], r + kBsellWindow));
+  }
+  brk_win.push_back(wins.size());
+  std::vector<int64_t> win_slice;
   std::vector<int64_t> sptr(1, 0);
   std::vector<int32_t> order;
-  for (int64_t w = 0; w < nwin; ++w) {
-    const int64_t r0 = w * kBsellWindow, r1 = std::min<int64_t>(N, r0 + kBsellWindow);
+  for (size_t w = 0; w < wins.size(); ++w) {
+    win_slice.push_back(int64_t(sptr.size()) - 1);
+    const int64_t r0 = wins[w].first, r1 = wins[w].second;
     order.resize(size_t(r1 - r0));
     for (int64_t R = r0; R < r1; ++R) order[size_t(R - r0)] = int32_t(R);
     std::stable_sort(order.begin(), order.end(),
@@ -164,6 +177,9 @@ int ensure_bsell(gf_ctx* ctx, gf_ell* m) {
   }
   m->bslices = int64_t(sptr.size()) - 1;
   m->bslots = sptr.back();
+  win_slice.push_back(m->bslices);
+  m->bsell_break_slice.clear();
+  for (size_t k = 0; k < brk_win.size(); ++k) m->bsell_break_slice.push_back(win_slice[brk_win[k]]);
   GF_HIP_CHECK(ctx, hipMalloc(&m->d_bperm, perm.size() * 4));
   GF_HIP_CHECK(ctx, hipMalloc(&m->d_bsptr, sptr.size() * 8));
   GF_HIP_CHECK(ctx, hipMalloc(&m->d_bcol, size_t(m->bslots) * 64 * 4));
