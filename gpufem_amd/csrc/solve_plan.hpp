@@ -44,8 +44,9 @@ struct SpmvOperands {
 SpmvOperands spmv_operands(const gf_ell* m, int variant);
 // builds what the variant reads and refreshes copies of stale values
 int prepare_variant(gf_ctx* ctx, gf_ell* m, int variant);
-// the node-block variant of the matrix's dimension, or -1
+// the node-block / block-sliced (swept) variant of the matrix's dimension, or -1
 int node_variant(const gf_ell* m);
+int bsell_variant(const gf_ell* m);
 int prepare_jacobi(gf_ctx* ctx, gf_ell* m);
 int run_pcg(gf_ctx* ctx, gf_ell* m, const PcgBuffers& B, int sched, int key, int use_x0,
             double rtol, int maxit, int verify, PcgHostState* h);
