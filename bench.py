@@ -4,11 +4,15 @@ A step is one pass of the hot path over the workload with inputs resident in
 HBM: numeric assembly (Ke + colored scatter into ELL), the penalty constraint,
 and a Jacobi-PCG solve from zero to ||b - K u|| <= 1e-8 ||b||.
 
-The default workload is the Q8 plate of 1024 x 512 elements (3,151,874 DOF,
-3x3 Gauss points).  `value` is assembled DOF/s (DOF / numeric assembly time);
-the `solve` object reports the PCG to 1e-8; `roofline` reports the ELL SpMV
-kernel the solver runs every iteration; `cpu_baseline` times the FP64 CPU
-restatement of the reference OpenMP path on the host cores.
+The default workload is the largest single-GPU configuration, the Brick8
+cube of 128^3 elements (6,440,067 DOF, 2x2x2 Gauss points, lognormal
+per-element moduli).  `value` is assembled DOF/s: DOF / numeric assembly
+time, the time covering every copy of the values the solver reads (the node
+gather writes the block-sliced copy beside the ELL values).  The `solve`
+object reports the PCG to 1e-8; `roofline` reports the SpMV kernel the
+solver runs every iteration; `cpu_baseline` times the FP64 CPU restatement of
+the reference OpenMP path on the host cores; `forms` times the mixed,
+batched, grouped and learned forms against plain FP64 PCG.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
@@ -35,18 +39,10 @@ def log(msg):
 
 def build_problem(name):
     from gpufem_amd import problems as P
-    if name == "q8_plate_1024x512":
-        return P.config_q8_plate(1024, 512)
-    if name == "q4_plate_64x32":
-        return P.config_q4_plate(64, 32)
-    if name == "brick8_cube_128":
-        return P.config_brick8_cube(128)
-    if name == "brick20_cube_48":
-        return P.config_brick20_cube(48)
-    if name.startswith("q8_plate_"):
-        nx, ny = map(int, name[len("q8_plate_"):].split("x"))
-        return P.config_q8_plate(nx, ny)
-    raise SystemExit("unknown workload %s" % name)
+    try:
+        return P.by_name(name)
+    except ValueError as e:
+        raise SystemExit(str(e))
 
 
 def ke_flops_per_element(dim, nen, ngp):
@@ -136,14 +132,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="q8_plate_1024x512")
+    ap.add_argument("--workload", default="brick8_cube_128")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=25.0)
     ap.add_argument("--schedule", default="graph",
                     choices=["kernels", "graph", "persistent"])
     ap.add_argument("--compare", default="all", choices=["all", "none"],
                     help="also time the other FP64 schedules and the mixed solve")
-    ap.add_argument("--forms", default="mixed,batched,grouped,learned,combined",
+    ap.add_argument("--forms", default="mixed,mixed_plate,batched,grouped,learned,combined",
                     help="comma list of forms timed against plain FP64 on their own "
                          "workloads (mixed, batched, grouped, learned) or 'none'")
     ap.add_argument("--weights", default=os.path.join(ROOT, "gpufem_amd", "weights",
@@ -293,8 +289,14 @@ def main():
     ms_spmv_b2b = C.c_double()
     gf.check(lib.gf_profile_spmv(ctx.handle, K.handle, 50, C.byref(ms_spmv_b2b)), ctx.handle)
     ms_spmv_pcg, ms_iter_pcg = ctx.profile_pcg(K, f, 300)
+    # algorithmic bytes of the ELL format (values, int32 columns, x, y)
     spmv_bytes = 12 * K.nnz + 16 * n
     achieved = spmv_bytes / (ms_spmv_pcg * 1e-3) / 1e9
+    # the least the block-sliced format must move: values, one 16-bit column
+    # offset per d x d block, x once, y once
+    d = p.dim
+    fmt_bytes = 8 * K.nnz + 2 * K.nnz // (d * d) + 16 * n
+    achieved_fmt = fmt_bytes / (ms_spmv_pcg * 1e-3) / 1e9
     # attainable streaming rate on this GPU: a plain copy kernel
     cs, cd = gf.DevBuf(ctx, COPY_BYTES), gf.DevBuf(ctx, COPY_BYTES)
     ms_copy = C.c_double()
@@ -353,6 +355,9 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(p.name, variant),
             "algorithmic_bytes_per_launch": spmv_bytes,
+            "format_bytes_per_launch": fmt_bytes,
+            "achieved_format_bytes": achieved_fmt,
+            "frac_format_bytes": achieved_fmt / HBM_PEAK_GBS,
             "ms_per_launch": ms_spmv_pcg,
             "ms_per_launch_back_to_back": ms_spmv_b2b.value,
             "ms_per_pcg_iteration_timed": ms_iter_pcg,
@@ -363,10 +368,18 @@ def main():
     }
     lib.gf_dev_free(ctx.handle, d_b)
     lib.gf_dev_free(ctx.handle, d_x)
-    del K, mesh
     if args.forms != "none":
         from gpufem_amd import forms
-        out["forms"] = forms.run(ctx, [f for f in args.forms.split(",") if f], args.weights)
+        # the mixed form runs on this workload when it is the Brick8 cube:
+        # the matrix is reassembled with the penalty, as a step leaves it
+        prepared = {}
+        if p.name.startswith("brick8_cube_"):
+            ctx.assemble(mesh, K)
+            ctx.apply_penalty(K, dofs)
+            prepared["mixed"] = (p, mesh, K)
+        out["forms"] = forms.run(ctx, [f for f in args.forms.split(",") if f], args.weights,
+                                 prepared)
+    del K, mesh
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(p, int(round(it_mean)), args.cpu_budget)

@@ -98,6 +98,12 @@ class Mesh:
         self.dim = ELEM_DIM[elem_type]
         self._arrays = arrays
 
+    def set_E(self, elem_E):
+        """New per-element moduli (the next assembly uses them)."""
+        eE = np.ascontiguousarray(elem_E, dtype=np.float64)
+        self._arrays = self._arrays[:2] + (eE,)
+        self.ctx._check(load().gf_mesh_set_E(self.ctx.handle, self.handle, _ptr(eE)))
+
     def free(self):
         if self.handle:
             load().gf_mesh_free(self.handle)

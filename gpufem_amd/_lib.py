@@ -59,7 +59,7 @@ class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("refinements", C.c_int32),
                 ("converged", C.c_int32), ("status", C.c_int32), ("relres", C.c_double),
                 ("true_relres", C.c_double), ("ms_device", C.c_double),
-                ("ms_host", C.c_double), ("fallbacks", C.c_int32), ("pad_", C.c_int32)]
+                ("ms_host", C.c_double), ("fallbacks", C.c_int32), ("x0_rejected", C.c_int32)]
 
 
 _P = C.c_void_p

@@ -220,6 +220,7 @@ static void ell_release(gf_ell* m) {
   dfree(m->d_col);
   dfree(m->d_coln);
   dfree(m->d_bperm);
+  dfree(m->d_binv);
   dfree(m->d_bsptr);
   dfree(m->d_bcol);
   dfree(m->d_bcoff);
@@ -509,7 +510,7 @@ static int assemble_with(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int varian
   // the gather writes every live slot; the scatters accumulate from zero
   if (variant != 3)
     GF_HIP_CHECK(ctx, hipMemsetAsync(m->d_val, 0, slots * sizeof(double), ctx->stream));
-  AssemblyArgs a;
+  AssemblyArgs a{};
   a.type = mesh->type;
   a.ngauss = mesh->ngauss;
   a.conn = mesh->d_conn;
@@ -526,6 +527,13 @@ static int assemble_with(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int varian
   a.n2e = m->d_n2e;
   a.rownnz = m->d_rownnz;
   a.nnodes = mesh->nnodes;
+  // a block-sliced copy of the values the solver reads is kept current
+  // inside the assembly (and its time): the node gather writes it beside
+  // the ELL values, the colored scatters are followed by the copy
+  const bool bsell = m->d_bval && m->bval_version != 0;
+  a.bval = bsell && variant == 3 ? m->d_bval : nullptr;
+  a.bsptr = m->d_bsptr;
+  a.binv = m->d_binv;
   int64_t launches = variant == 3 ? 0 : 1;
   if (variant == 3) {
     GF_HIP_CHECK(ctx, launch_assembly_gather(ctx->stream, a));
@@ -538,6 +546,11 @@ static int assemble_with(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int varian
       ++launches;
     }
   }
+  if (bsell && variant != 3) {
+    GF_HIP_CHECK(ctx, launch_bsell_vals(ctx->stream, bsell_view(m), m->dim, m->n, m->d_val,
+                                        m->d_rownnz, m->d_bval));
+    ++launches;
+  }
   double ms = 0;
   if ((st = record_stop(ctx, &ms))) return st;
   int bad = 0;
@@ -545,6 +558,7 @@ static int assemble_with(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int varian
   m->dinv_valid = false;
   m->val32_valid = false;
   ++m->val_version;
+  if (bsell) m->bval_version = m->val_version;
   if (t) {
     t->ms_device = ms;
     t->ms_host = now_ms() - h0;
@@ -626,6 +640,8 @@ int gf_apply_penalty(gf_ctx* ctx, gf_ell* m, const int32_t* dofs, int64_t nd, do
   }
   int st = select_device(ctx);
   if (st) return st;
+  // a current block-sliced copy gets the same diagonal values
+  const bool bsell = m->d_bval && m->bval_version == m->val_version;
   double* d_pen = reinterpret_cast<double*>(static_cast<char*>(ctx->d_state) + 192);
   GF_HIP_CHECK(ctx, launch_penalty_factor(ctx->stream, m->d_val, m->d_diag, m->n, coef,
                                           ctx->d_partials, 8192, d_pen));
@@ -638,7 +654,8 @@ int gf_apply_penalty(gf_ctx* ctx, gf_ell* m, const int32_t* dofs, int64_t nd, do
                          hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
       e = launch_penalty_apply(ctx->stream, m->d_val, m->d_diag, m->n, d_buf,
-                               d_buf + uniq.size(), int64_t(uniq.size()), d_pen);
+                               d_buf + uniq.size(), int64_t(uniq.size()), d_pen,
+                               bsell ? m->d_bval : nullptr, m->d_bsptr, m->d_binv, m->dim);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(d_buf);
     if (e != hipSuccess)
@@ -651,6 +668,7 @@ int gf_apply_penalty(gf_ctx* ctx, gf_ell* m, const int32_t* dofs, int64_t nd, do
   m->dinv_valid = false;
   m->val32_valid = false;
   ++m->val_version;
+  if (bsell) m->bval_version = m->val_version;
   return GF_OK;
 }
 

@@ -197,6 +197,17 @@ __device__ inline void load_node(const T* __restrict__ x, int64_t c, T* xv) {
 // U blocks per step (measured on the Q8 plate and the Brick8 cube: 4 for
 // 2 x 2 blocks, 3 for 3 x 3; fewer lose more in-flight loads than the extra
 // occupancy gives back, more cost occupancy).
+//
+// DIFF: the copy is in difference form (the FP32 copy of the mixed-precision
+// solve): an off-diagonal block K_RJ multiplies x_J - x_R and the diagonal
+// slot holds the block row sum S_R = sum_J K_RJ, so
+//   (K x)_R = sum_{J != R} K_RJ (x_J - x_R) + S_R x_R.
+// The stored operator then annihilates rigid translations exactly whatever
+// the rounding of its entries (the row sums are rounded once, not the sum of
+// rounded entries), which keeps an FP32 elasticity operator positive
+// definite: a plain FP32 copy of a structured mesh repeats the same rounding
+// in every row, which shifts every eigenvalue by about the same amount.
+// Padding slots point at the row's own node and hold zeros (adding 0).
 template <int D, class T, bool OFF>
 __device__ inline int64_t bsell_col(const int32_t* __restrict__ bcol,
                                     const int16_t* __restrict__ coff, int32_t base, int64_t b,
@@ -207,29 +218,44 @@ __device__ inline int64_t bsell_col(const int32_t* __restrict__ bcol,
     return int64_t(__builtin_nontemporal_load(bcol + b * 64 + lane)) * D;
 }
 
-template <int D, class T, bool OFF>
+template <int D, class T, bool OFF, bool DIFF = false>
 __device__ inline void bsell_row(const BsellView& bs, const T* __restrict__ bval,
                                  const int32_t* __restrict__ bcol, int64_t b0, int32_t cnt,
-                                 int lane, const T* __restrict__ x, T* acc, int32_t base) {
+                                 int lane, const T* __restrict__ x, T* acc, int32_t base,
+                                 int32_t row = 0) {
   constexpr int U = D == 2 ? 4 : 3;
 #pragma unroll
   for (int r = 0; r < D; ++r) acc[r] = T(0);
+  T xr[D];
+  if constexpr (DIFF) load_node<D, T>(x, int64_t(row) * D, xr);
+  // x_J - x_R off the diagonal, x_R on it
+  auto rel = [&](int64_t c, T* xv) {
+    if constexpr (DIFF) {
+      if (c != int64_t(row) * D) {
+#pragma unroll
+        for (int e = 0; e < D; ++e) xv[e] -= xr[e];
+      }
+    }
+  };
   int32_t j = 0;
   for (; j + U <= cnt; j += U) {
     T xv[U][D], v[U][D * D];
+    int64_t cu[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int64_t b = b0 + j + u;
-      const int64_t c = bsell_col<D, T, OFF>(bcol, bs.coff, base, b, lane);
+      cu[u] = bsell_col<D, T, OFF>(bcol, bs.coff, base, b, lane);
       load_block<D, T>(bval, b, lane, v[u]);
-      load_node<D, T>(x, c, xv[u]);
+      load_node<D, T>(x, cu[u], xv[u]);
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u)
+    for (int u = 0; u < U; ++u) {
+      rel(cu[u], xv[u]);
 #pragma unroll
       for (int r = 0; r < D; ++r)
 #pragma unroll
         for (int e = 0; e < D; ++e) acc[r] += v[u][r * D + e] * xv[u][e];
+    }
   }
   for (; j < cnt; ++j) {
     const int64_t b = b0 + j;
@@ -237,6 +263,7 @@ __device__ inline void bsell_row(const BsellView& bs, const T* __restrict__ bval
     T xv[D], v[D * D];
     load_block<D, T>(bval, b, lane, v);
     load_node<D, T>(x, c, xv);
+    rel(c, xv);
 #pragma unroll
     for (int r = 0; r < D; ++r)
 #pragma unroll
@@ -248,7 +275,7 @@ __device__ inline void bsell_row(const BsellView& bs, const T* __restrict__ bval
 // the SpMV variant V (thread per row with 1/4/8 slots in flight, 2/4 waves
 // sharing a tile of rows, node blocks, block-sliced slices).  Every thread of
 // the workgroup must call it.
-template <int V, class T, class F>
+template <int V, class T, bool DIFF = false, class F>
 __device__ inline void for_rows_spmv(int lb, int nlb, int64_t n, const T* __restrict__ val,
                                      const int32_t* __restrict__ col,
                                      const int32_t* __restrict__ rownnz,
@@ -268,8 +295,9 @@ __device__ inline void for_rows_spmv(int lb, int nlb, int64_t n, const T* __rest
       const int32_t w = int32_t(bs.sptr[sl + 1] - b0);
       const int32_t R = bs.perm[sl * 64 + lane];
       T acc[D];
-      if (bs.coff) bsell_row<D, T, true>(bs, val, col, b0, w, lane, x, acc, R >= 0 ? R : 0);
-      else bsell_row<D, T, false>(bs, val, col, b0, w, lane, x, acc, 0);
+      const int32_t Rn = R >= 0 ? R : 0;
+      if (bs.coff) bsell_row<D, T, true, DIFF>(bs, val, col, b0, w, lane, x, acc, Rn, Rn);
+      else bsell_row<D, T, false, DIFF>(bs, val, col, b0, w, lane, x, acc, 0, Rn);
       if (R < 0) continue;
 #pragma unroll
       for (int r = 0; r < D; ++r) f(int64_t(D) * R + r, acc[r]);

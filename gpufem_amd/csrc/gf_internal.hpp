@@ -78,6 +78,7 @@ struct gf_ell {
   int32_t* d_coln = nullptr;  // node columns for the node-block SpMV (lazily built)
   // block-sliced copy for the kBsell SpMV variants (solve.cpp; lazily built)
   int32_t* d_bperm = nullptr;
+  int32_t* d_binv = nullptr;  // node row -> its position in d_bperm
   int64_t* d_bsptr = nullptr;
   int64_t bslices = 0, bslots = 0;
   // node rows where block-sliced windows restart (a grouped batch: the first

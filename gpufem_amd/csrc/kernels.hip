@@ -439,6 +439,10 @@ struct GArgs {
   const int32_t* n2e;
   const int32_t* rownnz;
   int64_t nnodes;
+  // block-sliced copy written beside the ELL values (bval null: none)
+  double* bval;
+  const int64_t* bsptr;
+  const int32_t* binv;
 };
 
 template <int DIM, int NEN, int NGP, int MAXS, int RPT>
@@ -572,14 +576,27 @@ __global__ __launch_bounds__(64) void k_assemble_gather(GArgs ga) {
       }
     }
     if (live) {
+      // the block-sliced copy: block s of node row v sits at slot
+      // bsptr[slice] + s, lane pos & 63 of the slice, value (row, column)
+      // of the block at index row * DIM + column
+      int64_t bb0 = 0;
+      int bl = 0;
+      if (ga.bval) {
+        const int32_t pos = ga.binv[v];
+        bb0 = ga.bsptr[pos >> 6];
+        bl = pos & 63;
+      }
 #pragma unroll
       for (int r = 0; r < RPT; ++r) {
         const int64_t row = v * DIM + r0 + r;
         const int32_t ns = ga.rownnz[row] / DIM;
         for (int s = 0; s < ns; ++s)
 #pragma unroll
-          for (int j = 0; j < DIM; ++j)
-            a.val[row + (int64_t(s) * DIM + j) * a.n] = s_acc[((s * RPT + r) * DIM + j) * 64 + lane];
+          for (int j = 0; j < DIM; ++j) {
+            const double kv = s_acc[((s * RPT + r) * DIM + j) * 64 + lane];
+            a.val[row + (int64_t(s) * DIM + j) * a.n] = kv;
+            if (ga.bval) ga.bval[bsell_vidx(DIM * DIM, bb0 + s, (r0 + r) * DIM + j, bl)] = kv;
+          }
       }
     }
   }
@@ -657,7 +674,8 @@ __global__ __launch_bounds__(256) void k_max_finish(const double* partials, int 
 
 __global__ void k_penalty_apply(double* val, const int32_t* diag, int64_t n,
                                 const int32_t* dofs, const int32_t* counts, int64_t nd,
-                                const double* pen) {
+                                const double* pen, double* bval, const int64_t* bsptr,
+                                const int32_t* binv, int D) {
   for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < nd;
        t += int64_t(gridDim.x) * blockDim.x) {
     const int64_t d = dofs[t];
@@ -668,6 +686,12 @@ __global__ void k_penalty_apply(double* val, const int32_t* diag, int64_t n,
     const double p = *pen;
     for (int c = 0; c < counts[t]; ++c) x = p * x;
     *v = x;
+    if (bval) {
+      // ELL slot s of row d = D R + c is value (c, s % D) of block s / D of node row R
+      const int32_t pos = binv[d / D];
+      const int c = int(d % D);
+      bval[bsell_vidx(D * D, bsptr[pos >> 6] + s / D, c * D + s % D, pos & 63)] = x;
+    }
   }
 }
 
@@ -898,6 +922,9 @@ hipError_t launch_assembly_gather(hipStream_t s, const AssemblyArgs& a) {
   g.n2e = a.n2e;
   g.rownnz = a.rownnz;
   g.nnodes = a.nnodes;
+  g.bval = a.bval;
+  g.bsptr = a.bsptr;
+  g.binv = a.binv;
   const bool g3 = a.ngauss == 3;
   switch (a.type) {
     case GF_Q4: return g3 ? launch_gather<2, 4, 9, 9>(s, g) : launch_gather<2, 4, 4, 9>(s, g);
@@ -957,10 +984,11 @@ hipError_t launch_load_vector(hipStream_t s, int dim, int64_t n, const double* l
 
 hipError_t launch_penalty_apply(hipStream_t s, double* val, const int32_t* diag, int64_t n,
                                 const int32_t* dofs, const int32_t* counts, int64_t nd,
-                                const double* pen) {
+                                const double* pen, double* bval, const int64_t* bsptr,
+                                const int32_t* binv, int d) {
   if (nd == 0) return hipSuccess;
   hipLaunchKernelGGL(k_penalty_apply, dim3(grid_for(nd, 256, 4096)), dim3(256), 0, s, val,
-                     diag, n, dofs, counts, nd, pen);
+                     diag, n, dofs, counts, nd, pen, bval, bsptr, binv, d);
   return hipGetLastError();
 }
 

@@ -38,6 +38,11 @@ struct AssemblyArgs {
   const int32_t* n2e;
   const int32_t* rownnz;
   int64_t nnodes;
+  // node gather: also write the block-sliced copy of the values (null: no
+  // copy); bsptr = its slice pointers, binv = node row -> slice position
+  double* bval;
+  const int64_t* bsptr;
+  const int32_t* binv;
 };
 hipError_t launch_assembly_color(hipStream_t s, const AssemblyArgs& a);
 // Node-gather assembly (variant 3): one launch for the whole mesh, no
@@ -53,10 +58,13 @@ hipError_t launch_assembly_gather(hipStream_t s, const AssemblyArgs& a);
 hipError_t launch_penalty_factor(hipStream_t s, const double* val, const int32_t* diag,
                                  int64_t n, double coef, double* partials, int nparts,
                                  double* result);
-// K_dd <- pen^count_d * K_dd (applied as count repeated multiplications)
+// K_dd <- pen^count_d * K_dd (applied as count repeated multiplications);
+// with bval, the same value is stored into the block-sliced copy (d x d
+// blocks, slice pointers bsptr, node row -> slice position binv)
 hipError_t launch_penalty_apply(hipStream_t s, double* val, const int32_t* diag, int64_t n,
                                 const int32_t* dofs, const int32_t* counts, int64_t nd,
-                                const double* pen);
+                                const double* pen, double* bval, const int64_t* bsptr,
+                                const int32_t* binv, int d);
 
 // ---- zero substitution, right-hand sides, force vector --------------------------
 // fixed[d] = 1 for the listed DOFs (fixed zeroed by the caller); then every
@@ -197,7 +205,8 @@ int persistent_blocks(int variant, int num_cu);
 hipError_t launch_pcg_persistent(hipStream_t s, const PcgBuffers& B, void* barrier);
 // Host view of the scalar state.
 struct PcgHostState {
-  int32_t it, done, status, pad;
+  int32_t it, done, status;
+  int32_t x0_rejected;  // set by the host: a guarded initial guess was replaced by zero
   double rr, bnorm2, rho;
 };
 hipError_t read_pcg_state(hipStream_t s, const PcgBuffers& B, PcgHostState* h);

@@ -1,8 +1,11 @@
 // Host side of the mixed-precision solve (kernels in mixed.hip): FP32 copies
 // of the matrix values and of the Jacobi inverse, the FP32 work vectors, and
 // the chunked iteration loop.
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "mixed.hpp"
 #include "solve_plan.hpp"
@@ -12,6 +15,7 @@ struct gf_mixed_ws {
   float *dinv32 = nullptr, *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr;
   double* q64 = nullptr;
   double* part3 = nullptr;
+  double* hist = nullptr;  // alpha, beta per iteration (kLanczosIters pairs)
   gf::MixedState* st = nullptr;
   uint64_t dinv_version = 0;  // matrix values the FP32 Jacobi copy was made from
   hipGraphExec_t graph = nullptr;  // kChunk iterations, captured for graph_B
@@ -21,6 +25,56 @@ struct gf_mixed_ws {
 namespace gf {
 
 int set_error(gf_ctx* ctx, int code, const std::string& msg);
+
+namespace {
+// eigenvalues of the symmetric tridiagonal (d, e) below x (Sturm count)
+int sturm_below(const std::vector<double>& d, const std::vector<double>& e, double x) {
+  int cnt = 0;
+  double q = 1.0;
+  for (size_t j = 0; j < d.size(); ++j) {
+    const double off = j ? e[j - 1] * e[j - 1] : 0.0;
+    q = d[j] - x - (j ? off / q : 0.0);
+    if (q == 0.0) q = -1e-300;
+    if (q < 0.0) ++cnt;
+  }
+  return cnt;
+}
+}  // namespace
+
+double lanczos_condition(const double* alpha, const double* beta, int k, int stride) {
+  if (k < 2) return 0.0;
+  std::vector<double> d(static_cast<size_t>(k)), e(static_cast<size_t>(k - 1));
+  for (int j = 0; j < k; ++j) {
+    const double a = alpha[size_t(j) * stride];
+    if (!(a > 0.0)) return 0.0;
+    d[size_t(j)] = 1.0 / a;
+    if (j > 0) {
+      const double ap = alpha[size_t(j - 1) * stride], bp = beta[size_t(j - 1) * stride];
+      if (!(bp >= 0.0)) return 0.0;
+      d[size_t(j)] += bp / ap;
+      e[size_t(j - 1)] = std::sqrt(bp) / ap;
+    }
+  }
+  double hi = 0.0;  // Gershgorin bound
+  for (int j = 0; j < k; ++j)
+    hi = std::max(hi, d[size_t(j)] + (j ? e[size_t(j - 1)] : 0.0) + (j + 1 < k ? e[size_t(j)] : 0.0));
+  // smallest and largest eigenvalue by bisection, to 1e-3 relative
+  double lo = 0.0, up = hi;
+  for (int it = 0; it < 200 && up - lo > 1e-3 * up; ++it) {
+    const double mid = 0.5 * (lo + up);
+    if (sturm_below(d, e, mid) >= 1) up = mid;
+    else lo = mid;
+  }
+  const double lmin = up;
+  double lo2 = 0.0, up2 = hi;
+  for (int it = 0; it < 200 && up2 - lo2 > 1e-3 * up2; ++it) {
+    const double mid = 0.5 * (lo2 + up2);
+    if (sturm_below(d, e, mid) >= k) up2 = mid;
+    else lo2 = mid;
+  }
+  if (!(lmin > 0.0)) return 0.0;
+  return up2 / lmin;
+}
 
 namespace {
 constexpr int kChunk = 32;
@@ -56,8 +110,8 @@ int ensure_val32(gf_ctx* ctx, gf_ell* m, int variant, const float** out) {
     const size_t cnt = size_t(m->bslots) * 64 * m->dim * m->dim;
     if (!m->d_bval32) GF_HIP_CHECK(ctx, hipMalloc(&m->d_bval32, cnt * sizeof(float)));
     if (m->bval32_version != m->val_version) {
-      GF_HIP_CHECK(ctx, launch_bsell_to_f32(ctx->stream, m->dim * m->dim, m->bslots, m->d_bval,
-                                            m->d_bval32));
+      GF_HIP_CHECK(ctx, launch_bsell_to_f32(ctx->stream, m->dim, bsell_view(m), m->d_bcol,
+                                            m->d_bval, m->d_bval32));
       m->bval32_version = m->val_version;
     }
     *out = m->d_bval32;
@@ -81,6 +135,7 @@ void release_mixed(gf_solve_plan* P) {
     if (v) (void)hipFree(v);
   if (W->q64) (void)hipFree(W->q64);
   if (W->part3) (void)hipFree(W->part3);
+  if (W->hist) (void)hipFree(W->hist);
   if (W->st) (void)hipFree(W->st);
   if (W->graph) (void)hipGraphExecDestroy(W->graph);
   delete W;
@@ -102,6 +157,7 @@ int run_mixed(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, PcgHostState* h
       if (e == hipSuccess) e = hipMalloc(v, fb);
     if (e == hipSuccess) e = hipMalloc(&W->q64, size_t(n) * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&W->part3, 3 * kMaxPcgBlocks * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&W->hist, 2 * size_t(kLanczosIters) * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&W->st), mixed_state_bytes());
     if (e != hipSuccess) {
       release_mixed(P);
@@ -140,6 +196,7 @@ int run_mixed(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, PcgHostState* h
   B.part2 = P->part2;
   B.part3 = W->part3;
   B.st = W->st;
+  B.hist = W->hist;
   GF_HIP_CHECK(ctx, launch_mixed_setup(ctx->stream, B, opts.use_x0, opts.rtol, kDelta, opts.maxit));
   MixedHostState hs{};
   GF_HIP_CHECK(ctx, read_mixed_state(ctx->stream, B, &hs));
@@ -147,6 +204,9 @@ int run_mixed(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, PcgHostState* h
   // one by one); the results are identical
   const bool graph = opts.schedule != GF_SCHED_KERNELS;
   if (graph && !hs.done) GF_HIP_CHECK(ctx, ensure_mixed_graph(ctx, W, B));
+  std::vector<double> hist;
+  int checked = 0;
+  bool ill = false;
   while (!hs.done) {
     if (graph) {
       GF_HIP_CHECK(ctx, hipGraphLaunch(W->graph, ctx->stream));
@@ -155,7 +215,22 @@ int run_mixed(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, PcgHostState* h
         GF_HIP_CHECK(ctx, launch_mixed_iteration(ctx->stream, B, it & 1));
     }
     GF_HIP_CHECK(ctx, read_mixed_state(ctx->stream, B, &hs));
+    // condition check: continue in FP64 once FP32 cannot resolve the
+    // operator's smallest eigenvalues
+    const int k = std::min(hs.it, kLanczosIters);
+    if (!hs.done && k >= checked + kLanczosCheck) {
+      hist.resize(2 * size_t(k));
+      GF_HIP_CHECK(ctx, hipMemcpy(hist.data() + 2 * size_t(checked), W->hist + 2 * size_t(checked),
+                                  2 * size_t(k - checked) * sizeof(double), hipMemcpyDeviceToHost));
+      checked = k;
+      const double kappa = lanczos_condition(hist.data(), hist.data() + 1, k, 2);
+      if (kappa * kU32 > kKappaU32) {
+        ill = true;
+        break;
+      }
+    }
   }
+  if (ill) hs.status = kMixedStalled;
   h->it = hs.it;
   h->done = hs.done;
   h->status = hs.status;

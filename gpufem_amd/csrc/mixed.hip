@@ -28,7 +28,9 @@ struct MixedState {
   double bnorm2, rtol2, delta2;
   double rr_rec;   // recursive ||r||^2 after the last update
   double rr_true;  // ||b - K x||^2 at the last refresh
-  int32_t it, done, status, maxit, refinements, pad;
+  int32_t it, done, status, maxit, refinements;
+  int32_t stall;  // consecutive refreshes that found the true residual less than halved
+  double alpha_last;  // alpha of the current iteration (history)
 };
 
 namespace {
@@ -41,14 +43,32 @@ __global__ void k_to_f32(int64_t count, const double* __restrict__ in, float* __
     out[i] = float(in[i]);
 }
 
-// FP32 copy of the block-sliced values, from the FP64 pairs to FP32 quads
-__global__ void k_bsell_to_f32(int dd, int64_t slots, const double* __restrict__ in,
-                               float* __restrict__ out) {
-  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < slots * 64;
-       t += int64_t(gridDim.x) * blockDim.x) {
-    const int64_t b = t >> 6;
-    const int l = int(t & 63);
-    for (int k = 0; k < dd; ++k) out[bsell_vidx32(dd, b, k, l)] = float(in[bsell_vidx(dd, b, k, l)]);
+// FP32 copy of the block-sliced values in difference form (FP64 pairs to
+// FP32 quads): the off-diagonal blocks rounded, the diagonal slot (the first
+// slot whose node column is the row's node) replaced by the block row sum
+// S_R = sum_J K_RJ, summed in FP64 in slot order and rounded once.  One
+// thread per slice position (node row).
+__global__ void k_bsell_to_f32(int D, BsellView bs, const int32_t* __restrict__ bcol,
+                               const double* __restrict__ in, float* __restrict__ out) {
+  const int dd = D * D;
+  for (int64_t pos = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; pos < bs.nslices * 64;
+       pos += int64_t(gridDim.x) * blockDim.x) {
+    const int l = int(pos & 63);
+    const int32_t R = bs.perm[pos];
+    const int64_t b0 = bs.sptr[pos >> 6], b1 = bs.sptr[(pos >> 6) + 1];
+    double S[9] = {};
+    int64_t bdiag = -1;
+    for (int64_t b = b0; b < b1; ++b) {
+      const bool diag = R >= 0 && bdiag < 0 && bcol[b * 64 + l] == R;
+      if (diag) bdiag = b;
+      for (int k = 0; k < dd; ++k) {
+        const double v = in[bsell_vidx(dd, b, k, l)];
+        S[k] += v;
+        out[bsell_vidx32(dd, b, k, l)] = float(v);
+      }
+    }
+    if (bdiag >= 0)
+      for (int k = 0; k < dd; ++k) out[bsell_vidx32(dd, bdiag, k, l)] = float(S[k]);
   }
 }
 
@@ -104,6 +124,7 @@ __global__ __launch_bounds__(kNT) void k_m_init_finish(MixedBuffers B, int nb, d
     st->status = GF_OK;
     st->maxit = maxit;
     st->refinements = 0;
+    st->stall = 0;
     st->done = (rr <= rtol2 * bb) || maxit <= 0;
   }
 }
@@ -113,7 +134,7 @@ __global__ __launch_bounds__(kNT) void k_m_spmv_pq(MixedBuffers B) {
   if (B.st->done) return;
   __shared__ double sh[kNT / 64];
   double pq = 0.0;
-  for_rows_spmv<V, float>(logical_block(), gridDim.x, B.n, B.val32, B.col, B.rownnz, B.p32,
+  for_rows_spmv<V, float, true>(logical_block(), gridDim.x, B.n, B.val32, B.col, B.rownnz, B.p32,
                           [&](int64_t i, float qi) {
                             B.q32[i] = qi;
                             pq += double(B.p32[i]) * double(qi);
@@ -135,6 +156,7 @@ __global__ __launch_bounds__(kNT) void k_m_update(MixedBuffers B, int parity) {
     return;
   }
   const double alpha = st->rho[parity] / pq;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha_last = alpha;
   double rz = 0, rr = 0;
   for_my_rows(B.n, [&](int64_t i) {
     const double pi = double(B.p32[i]);
@@ -211,14 +233,26 @@ __global__ __launch_bounds__(kNT) void k_m_direction(MixedBuffers B, int parity)
   }
   __syncthreads();
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // a refresh that finds the true residual less than halved since the
+    // previous one means the FP32 operator no longer resolves the error
+    // that is left; after kStallRefreshes of them in a row the FP32
+    // iteration stops (status kMixedStalled) and the host continues in FP64
+    int stall = st->stall;
+    if (rel && !converged) stall = rr_true > 0.25 * st->rref2[parity] ? stall + 1 : 0;
+    st->stall = stall;
     st->rho[parity ^ 1] = rz;
     st->rref2[parity ^ 1] = rel ? rr_true : st->rref2[parity];
     st->rr_rec = rel ? rr_true : rr;
     st->rr_true = rr_true;
     st->refinements += rel ? 1 : 0;
     const int it = st->it + 1;
+    if (B.hist && it <= kLanczosIters) {
+      B.hist[2 * (it - 1)] = st->alpha_last;
+      B.hist[2 * (it - 1) + 1] = converged ? 0.0 : rz / st->rho[parity];
+    }
     st->it = it;
-    if (converged || it >= st->maxit) st->done = 1;
+    if (stall >= kStallRefreshes && !converged) st->status = kMixedStalled;
+    if (converged || it >= st->maxit || stall >= kStallRefreshes) st->done = 1;
   }
 }
 
@@ -239,10 +273,10 @@ hipError_t launch_to_f32(hipStream_t s, int64_t count, const double* in, float* 
   return hipGetLastError();
 }
 
-hipError_t launch_bsell_to_f32(hipStream_t s, int dd, int64_t slots, const double* in,
-                              float* out) {
-  hipLaunchKernelGGL(k_bsell_to_f32, dim3(grid_for(slots * 64, 256, 16384)), dim3(256), 0, s, dd,
-                     slots, in, out);
+hipError_t launch_bsell_to_f32(hipStream_t s, int d, const BsellView& bs, const int32_t* bcol,
+                               const double* in, float* out) {
+  hipLaunchKernelGGL(k_bsell_to_f32, dim3(grid_for(bs.nslices * 64, 256, 16384)), dim3(256), 0, s,
+                     d, bs, bcol, in, out);
   return hipGetLastError();
 }
 

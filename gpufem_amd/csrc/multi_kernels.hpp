@@ -197,7 +197,10 @@ __device__ inline void ell_row_chunk(const SpmmArgs& A, const VT* __restrict__ v
 
 // Block-sliced copy: positions of the node-row permutation in grid-stride
 // order, G per workgroup pass; f(R, c0, acc) gets the D rows of node row R.
-template <int S, int D, class T, class VT, class F>
+// DIFF: the values are the difference form of the FP32 copy (see bsell_row):
+// off-diagonal blocks multiply x_J - x_R, the diagonal slot (node column R)
+// holds the block row sum and multiplies x_R.
+template <int S, int D, class T, class VT, bool DIFF = false, class F>
 __device__ inline void for_chunk_node_rows(const SpmmArgs& A, const VT* __restrict__ val,
                                            const T* __restrict__ xin, F&& f) {
   int g, c0;
@@ -215,26 +218,46 @@ __device__ inline void for_chunk_node_rows(const SpmmArgs& A, const VT* __restri
     for (int r = 0; r < D; ++r)
 #pragma unroll
       for (int j = 0; j < kC; ++j) acc[r][j] = T(0);
+    T xr[D][kC];
+    if constexpr (DIFF) {
+#pragma unroll
+      for (int e = 0; e < D; ++e) load_chunk(xin + (int64_t(D) * R + e) * S + c0, xr[e]);
+    }
+    // x_J - x_R off the diagonal (DIFF)
+    auto rel = [&](int64_t c, T (*xv)[kC]) {
+      if constexpr (DIFF) {
+        if (c != int64_t(D) * R) {
+#pragma unroll
+          for (int e = 0; e < D; ++e)
+#pragma unroll
+            for (int j = 0; j < kC; ++j) xv[e][j] -= xr[e][j];
+        }
+      }
+    };
     int32_t jb = 0;
     for (; jb + 2 <= cnt; jb += 2) {
       T v[2][D * D], xv[2][D][kC];
+      int64_t cu[2];
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const int64_t b = b0 + jb + u;
         const int64_t c = int64_t(A.col[b * 64 + lp]) * D;
+        cu[u] = c;
 #pragma unroll
         for (int k = 0; k < D * D; ++k) v[u][k] = T(val[bsell_index<VT>(D * D, b, k, lp)]);
 #pragma unroll
         for (int e = 0; e < D; ++e) load_chunk(xin + (c + e) * S + c0, xv[u][e]);
       }
 #pragma unroll
-      for (int u = 0; u < 2; ++u)
+      for (int u = 0; u < 2; ++u) {
+        rel(cu[u], xv[u]);
 #pragma unroll
         for (int r = 0; r < D; ++r)
 #pragma unroll
           for (int e = 0; e < D; ++e)
 #pragma unroll
             for (int j = 0; j < kC; ++j) acc[r][j] += v[u][r * D + e] * xv[u][e][j];
+      }
     }
     if (jb < cnt) {
       const int64_t b = b0 + jb;
@@ -244,6 +267,7 @@ __device__ inline void for_chunk_node_rows(const SpmmArgs& A, const VT* __restri
       for (int k = 0; k < D * D; ++k) v[k] = T(val[bsell_index<VT>(D * D, b, k, lp)]);
 #pragma unroll
       for (int e = 0; e < D; ++e) load_chunk(xin + (c + e) * S + c0, xv[e]);
+      rel(c, xv);
 #pragma unroll
       for (int r = 0; r < D; ++r)
 #pragma unroll

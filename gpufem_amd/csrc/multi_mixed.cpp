@@ -1,8 +1,10 @@
 // Host driver of the batched mixed-precision PCG (kernels in
 // multi_mixed.hip): workspace, FP32 copies, and the chunked iteration loop
 // (replayed from a captured hipGraph unless the kernel schedule is asked).
+#include <algorithm>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "mixed.hpp"
 #include "multi.hpp"
@@ -15,6 +17,7 @@ struct gf_mmulti_ws {
   double *b = nullptr, *x = nullptr;
   float *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *dinv32 = nullptr;
   double *part1 = nullptr, *part2 = nullptr, *part3 = nullptr;
+  double* hist = nullptr;  // alpha, beta per column and iteration (kLanczosIters)
   void* st = nullptr;
   uint64_t dinv_version = ~0ull;  // matrix values the FP32 Jacobi copy was made from
   hipGraphExec_t graph = nullptr;  // kChunk iterations, captured for graph_B
@@ -55,7 +58,8 @@ hipError_t ensure_graph(gf_ctx* ctx, gf_mmulti_ws* W, const MixedMultiBuffers& B
 void release_multi_mixed(gf_solve_plan* P) {
   gf_mmulti_ws* W = P->mmulti;
   if (!W) return;
-  void* v[] = {W->b, W->x, W->r, W->z, W->p, W->q, W->dinv32, W->part1, W->part2, W->part3, W->st};
+  void* v[] = {W->b,     W->x,     W->r,     W->z,     W->p,    W->q,
+               W->dinv32, W->part1, W->part2, W->part3, W->hist, W->st};
   for (void* d : v)
     if (d) (void)hipFree(d);
   if (W->graph) (void)hipGraphExecDestroy(W->graph);
@@ -86,6 +90,8 @@ int run_multi_mixed(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, const dou
     if (e == hipSuccess) e = hipMalloc(&W->part1, size_t(S) * nb * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&W->part2, size_t(3 * S) * nb * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&W->part3, size_t(2 * S) * nb * sizeof(double));
+    if (e == hipSuccess)
+      e = hipMalloc(&W->hist, 2 * size_t(S) * size_t(kLanczosIters) * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&W->st, mixed_multi_state_bytes(S));
     if (e != hipSuccess) {
       release_multi_mixed(P);
@@ -127,6 +133,7 @@ int run_multi_mixed(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, const dou
   B.part2 = W->part2;
   B.part3 = W->part3;
   B.st = W->st;
+  B.hist = W->hist;
   GF_HIP_CHECK(ctx, launch_multi_pack(ctx->stream, S, n, ncols, Bd, W->b));
   if (opts.use_x0) {
     GF_HIP_CHECK(ctx, launch_multi_pack(ctx->stream, S, n, ncols, Xd, W->x));
@@ -137,6 +144,10 @@ int run_multi_mixed(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, const dou
   GF_HIP_CHECK(ctx, read_mixed_multi_state(ctx->stream, B, h));
   const bool graph = opts.schedule != GF_SCHED_KERNELS;
   if (graph && !h->done) GF_HIP_CHECK(ctx, ensure_graph(ctx, W, B));
+  // condition check of every column, as the single-RHS mixed solve does
+  std::vector<double> hist;
+  int checked = 0;
+  h->ill = 0;
   while (!h->done) {
     if (graph) {
       GF_HIP_CHECK(ctx, hipGraphLaunch(W->graph, ctx->stream));
@@ -145,6 +156,22 @@ int run_multi_mixed(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, const dou
         GF_HIP_CHECK(ctx, launch_mixed_multi_iteration(ctx->stream, B));
     }
     GF_HIP_CHECK(ctx, read_mixed_multi_state(ctx->stream, B, h));
+    const int k = std::min(h->it, kLanczosIters);
+    if (!h->done && k >= checked + kLanczosCheck) {
+      hist.resize(2 * size_t(S) * size_t(k));
+      const size_t o = 2 * size_t(S) * size_t(checked);
+      GF_HIP_CHECK(ctx, hipMemcpy(hist.data() + o, W->hist + o,
+                                  2 * size_t(S) * size_t(k - checked) * sizeof(double),
+                                  hipMemcpyDeviceToHost));
+      checked = k;
+      for (int c = 0; c < ncols && !h->ill; ++c) {
+        const int kc = std::min(h->col_it[c], k);
+        const double kappa = lanczos_condition(hist.data() + 2 * c, hist.data() + 2 * c + 1, kc,
+                                               2 * S);
+        if (kappa * kU32 > kKappaU32) h->ill = 1;
+      }
+      if (h->ill) break;
+    }
   }
   for (int k = 0; k < ncols; ++k) {
     const double bb = h->bnorm2[k];

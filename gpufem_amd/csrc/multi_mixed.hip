@@ -19,6 +19,7 @@
 //   k_mm_direction p = z + beta p
 #include "gf_internal.hpp"
 #include "launch.hpp"
+#include "mixed.hpp"
 #include "multi_mixed.hpp"
 #include "multi_kernels.hpp"
 
@@ -32,6 +33,7 @@ struct MMState {
   double rr_true[S]; // ||b - K x||^2 at the last refresh
   double rz_new[S];  // r.z after the last update
   int32_t col_it[S], col_done[S], col_status[S], col_refresh[S], col_refinements[S];
+  int32_t col_stall[S];  // consecutive refreshes that less than halved the true residual
   int32_t it, done, status, maxit, any_refresh, pad;
   double rtol2, delta2;
 };
@@ -133,6 +135,7 @@ __global__ __launch_bounds__(kNT) void k_mm_setup_finish(MixedMultiBuffers B, in
     st->col_status[k] = GF_OK;
     st->col_refresh[k] = 0;
     st->col_refinements[k] = 0;
+    st->col_stall[k] = 0;
     st->col_done[k] = (k >= ncols) || (rr[k] <= rtol2 * bb[k]) || maxit <= 0;
   }
   __syncthreads();
@@ -170,7 +173,7 @@ __global__ __launch_bounds__(kNT) void k_mm_spmm_pq(MixedMultiBuffers B) {
       row(i, c0, acc);
     });
   } else {
-    for_chunk_node_rows<S, D>(spmm_args(B), B.val32, B.p,
+    for_chunk_node_rows<S, D, float, float, true>(spmm_args(B), B.val32, B.p,
                               [&](int32_t R, int c0, const float (*acc)[kC]) {
 #pragma unroll
                                 for (int r = 0; r < D; ++r) row(int64_t(D) * R + r, c0, acc[r]);
@@ -289,15 +292,28 @@ __global__ __launch_bounds__(kNT) void k_mm_beta(MixedMultiBuffers B, int nb) {
     double beta = 0.0;
     if (!st->col_done[k]) {
       double rz = st->rz_new[k];
+      bool stalled = false;
       if (st->col_refresh[k]) {
         rz = rz2[k];
+        // as the single-RHS solve: kStallRefreshes refreshes in a row that
+        // less than halve the true residual stop this column's FP32 iteration
+        const bool met = rr2[k] <= st->rtol2 * st->bnorm2[k];
+        st->col_stall[k] = !met && rr2[k] > 0.25 * st->rref2[k] ? st->col_stall[k] + 1 : 0;
+        stalled = st->col_stall[k] >= kStallRefreshes;
         st->rr_true[k] = rr2[k];
         st->rref2[k] = rr2[k];
         st->col_refinements[k] += 1;
       }
       st->col_it[k] = it;
+      if (B.hist && it <= kLanczosIters) {
+        B.hist[2 * (int64_t(S) * (it - 1) + k)] = st->alpha[k];
+        B.hist[2 * (int64_t(S) * (it - 1) + k) + 1] = rz / st->rho[k];
+      }
       if (st->col_refresh[k] && st->rr_true[k] <= st->rtol2 * st->bnorm2[k]) {
         st->col_done[k] = 1;
+      } else if (stalled) {
+        st->col_done[k] = 1;
+        st->col_status[k] = kMixedStalled;
       } else if (it >= st->maxit) {
         st->col_done[k] = 1;
         st->col_status[k] = GF_ERR_NOCONV;

@@ -30,10 +30,14 @@ struct MixedMultiBuffers {
   double* part2;  // [3S][nblocks]  r.z, r.r (and b.b at setup)
   double* part3;  // [2S][nblocks]  refreshed r.z, r.r
   void* st;       // MMState<S>
+  // alpha, beta of every column for iterations 1..kLanczosIters:
+  // hist[2 (S (it - 1) + k) + {0, 1}], for the host's condition estimate
+  double* hist;
 };
 
 struct MixedMultiHostState {
   int32_t it, done;
+  int32_t ill;  // set by the host: some column's FP32 operator too ill-conditioned
   int32_t col_it[64], col_status[64], col_refinements[64];
   double rr_true[64], rr_rec[64], bnorm2[64];
 };

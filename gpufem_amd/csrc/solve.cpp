@@ -89,17 +89,21 @@ int ensure_plan(gf_ctx* ctx, gf_ell* m) {
 
 // Fixes the variant and the PCG grid (the same grid for every schedule, so
 // all schedules give bitwise-identical results).
+int pcg_grid(int variant, int64_t n, int num_cu) {
+  int nb = spmv_blocks(n);
+  const int pc = persistent_blocks(variant, num_cu);
+  if (pc > 0 && nb > pc) nb = pc;
+  if (nb >= 8) nb &= ~7;
+  return std::max(1, nb);
+}
+
 void set_variant(gf_ell* m, int variant) {
   gf_solve_plan* P = m->plan;
   if (P->variant == variant && P->nblocks > 0) return;
   drop_graphs(P);
   P->variant = variant;
-  int nb = spmv_blocks(m->n);
-  const int pc = persistent_blocks(variant, P->num_cu);
-  P->persistent_ok = pc > 0;
-  if (pc > 0 && nb > pc) nb = pc;
-  if (nb >= 8) nb &= ~7;
-  P->nblocks = std::max(1, nb);
+  P->persistent_ok = persistent_blocks(variant, P->num_cu) > 0;
+  P->nblocks = pcg_grid(variant, m->n, P->num_cu);
 }
 
 // The node-block / block-sliced variants of the matrix's dimension, or -1
@@ -175,35 +179,60 @@ int ensure_bsell(gf_ctx* ctx, gf_ell* m) {
       sptr.push_back(sptr.back() + cnt[size_t(order[s0])]);
     }
   }
-  m->bslices = int64_t(sptr.size()) - 1;
-  m->bslots = sptr.back();
-  win_slice.push_back(m->bslices);
-  m->bsell_break_slice.clear();
-  for (size_t k = 0; k < brk_win.size(); ++k) m->bsell_break_slice.push_back(win_slice[brk_win[k]]);
-  GF_HIP_CHECK(ctx, hipMalloc(&m->d_bperm, perm.size() * 4));
-  GF_HIP_CHECK(ctx, hipMalloc(&m->d_bsptr, sptr.size() * 8));
-  GF_HIP_CHECK(ctx, hipMalloc(&m->d_bcol, size_t(m->bslots) * 64 * 4));
-  GF_HIP_CHECK(ctx, hipMalloc(&m->d_bval, size_t(m->bslots) * 64 * d * d * sizeof(double)));
-  GF_HIP_CHECK(ctx, hipMemcpy(m->d_bperm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice));
-  GF_HIP_CHECK(ctx, hipMemcpy(m->d_bsptr, sptr.data(), sptr.size() * 8, hipMemcpyHostToDevice));
-  GF_HIP_CHECK(ctx, launch_bsell_cols(ctx->stream, bsell_view(m), d, m->n, m->d_col, m->d_rownnz,
-                                      m->d_bcol));
+  win_slice.push_back(int64_t(sptr.size()) - 1);
+  std::vector<int32_t> inv(static_cast<size_t>(N), -1);
+  for (size_t pos = 0; pos < perm.size(); ++pos)
+    if (perm[pos] >= 0) inv[size_t(perm[pos])] = int32_t(pos);
+  // every array is allocated and filled before the copy is published
+  // (m->d_bperm set last): a failure leaves no half-built copy behind
+  struct Arrays {
+    int32_t *perm = nullptr, *inv = nullptr, *col = nullptr;
+    int64_t* sptr = nullptr;
+    int16_t* coff = nullptr;
+    double* val = nullptr;
+    ~Arrays() {
+      for (void* p : {static_cast<void*>(perm), static_cast<void*>(inv), static_cast<void*>(col),
+                      static_cast<void*>(sptr), static_cast<void*>(coff), static_cast<void*>(val)})
+        if (p) (void)hipFree(p);
+    }
+  } A;
+  const int64_t bslices = int64_t(sptr.size()) - 1, bslots = sptr.back();
+  GF_HIP_CHECK(ctx, hipMalloc(&A.perm, perm.size() * 4));
+  GF_HIP_CHECK(ctx, hipMalloc(&A.inv, std::max<size_t>(inv.size(), 1) * 4));
+  GF_HIP_CHECK(ctx, hipMalloc(&A.sptr, sptr.size() * 8));
+  GF_HIP_CHECK(ctx, hipMalloc(&A.col, size_t(bslots) * 64 * 4));
+  GF_HIP_CHECK(ctx, hipMalloc(&A.val, size_t(bslots) * 64 * d * d * sizeof(double)));
+  GF_HIP_CHECK(ctx, hipMalloc(&A.coff, size_t(bslots) * 64 * sizeof(int16_t)));
+  GF_HIP_CHECK(ctx, hipMemcpy(A.perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice));
+  GF_HIP_CHECK(ctx, hipMemcpy(A.inv, inv.data(), inv.size() * 4, hipMemcpyHostToDevice));
+  GF_HIP_CHECK(ctx, hipMemcpy(A.sptr, sptr.data(), sptr.size() * 8, hipMemcpyHostToDevice));
+  const BsellView v0{A.perm, A.sptr, bslices, nullptr};
+  GF_HIP_CHECK(ctx, launch_bsell_cols(ctx->stream, v0, d, m->n, m->d_col, m->d_rownnz, A.col));
   // 16-bit node-column offsets for the single-RHS kernels when they all fit
-  GF_HIP_CHECK(ctx, hipMalloc(&m->d_bcoff, size_t(m->bslots) * 64 * sizeof(int16_t)));
   int* d_over = reinterpret_cast<int*>(static_cast<char*>(ctx->d_state) + 160);
   GF_HIP_CHECK(ctx, hipMemsetAsync(d_over, 0, sizeof(int), ctx->stream));
-  {
-    BsellView v = bsell_view(m);
-    v.coff = nullptr;
-    GF_HIP_CHECK(ctx, launch_bsell_offsets(ctx->stream, v, m->d_bcol, m->d_bcoff, d_over));
-  }
+  GF_HIP_CHECK(ctx, launch_bsell_offsets(ctx->stream, v0, A.col, A.coff, d_over));
   int over = 0;
   GF_HIP_CHECK(ctx, hipMemcpyAsync(&over, d_over, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   GF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   if (over) {
-    GF_HIP_CHECK(ctx, hipFree(m->d_bcoff));
-    m->d_bcoff = nullptr;
+    GF_HIP_CHECK(ctx, hipFree(A.coff));
+    A.coff = nullptr;
   }
+  m->bslices = bslices;
+  m->bslots = bslots;
+  m->bsell_break_slice.clear();
+  for (size_t k = 0; k < brk_win.size(); ++k) m->bsell_break_slice.push_back(win_slice[brk_win[k]]);
+  m->d_binv = A.inv;
+  m->d_bsptr = A.sptr;
+  m->d_bcol = A.col;
+  m->d_bcoff = A.coff;
+  m->d_bval = A.val;
+  m->d_bperm = A.perm;
+  A.perm = A.inv = A.col = nullptr;
+  A.sptr = nullptr;
+  A.coff = nullptr;
+  A.val = nullptr;
   m->bval_version = 0;
   m->bval32_version = 0;
   return GF_OK;
@@ -318,8 +347,19 @@ int run_pcg(gf_ctx* ctx, gf_ell* m, const PcgBuffers& B, int sched, int key, int
             double rtol, int maxit, int verify, PcgHostState* h) {
   gf_solve_plan* P = m->plan;
   int total = 0;
+  bool set_up = false;
+  if (use_x0 == 2) {
+    // guarded initial guess: kept only when ||b - K x0|| < ||b||, otherwise
+    // the solve starts from zero (the setup's own SpMV is the check)
+    GF_HIP_CHECK(ctx, launch_pcg_setup(ctx->stream, B, 1, rtol, maxit));
+    GF_HIP_CHECK(ctx, read_pcg_state(ctx->stream, B, h));
+    set_up = h->rr < h->bnorm2;
+    use_x0 = set_up ? 1 : 0;
+    h->x0_rejected = set_up ? 0 : 1;
+  }
   for (int restart = 0;; ++restart) {
-    GF_HIP_CHECK(ctx, launch_pcg_setup(ctx->stream, B, use_x0, rtol, maxit - total));
+    if (!set_up) GF_HIP_CHECK(ctx, launch_pcg_setup(ctx->stream, B, use_x0, rtol, maxit - total));
+    set_up = false;
     if (sched == GF_SCHED_PERSISTENT) {
       GF_HIP_CHECK(ctx, launch_pcg_persistent(ctx->stream, B, P->barrier));
       GF_HIP_CHECK(ctx, read_pcg_state(ctx->stream, B, h));
@@ -434,14 +474,6 @@ int gf_pcg(gf_ctx* ctx, gf_ell* m, const double* b, double* x, int32_t nrhs,
   if (P->variant < 0 && (st = tune_spmv(ctx, m, -1, nullptr, nullptr))) return st;
   const int64_t n = m->n;
   const size_t vb = size_t(n) * sizeof(double);
-  if (opts.precond == GF_PRECOND_JACOBI || opts.precision == GF_PREC_MIXED) {
-    st = prepare_jacobi(ctx, m);
-    if (st) return st;
-  }
-  const double* dinv = opts.precond == GF_PRECOND_JACOBI ? m->d_dinv : nullptr;
-  const int key = dinv ? 1 : 0;
-  if ((st = prepare_variant(ctx, m, P->variant))) return st;
-  PcgBuffers B = make_buffers(m, dinv);
   int sched = opts.schedule == GF_SCHED_AUTO ? GF_SCHED_GRAPH : opts.schedule;
   if (sched == GF_SCHED_PERSISTENT && !P->persistent_ok)
     return set_error(ctx, GF_ERR_UNSUPPORTED, "gf_pcg: persistent kernel cannot be co-resident");
@@ -463,19 +495,35 @@ int gf_pcg(gf_ctx* ctx, gf_ell* m, const double* b, double* x, int32_t nrhs,
     Bd = tmp.p;
     Xd = tmp.p + n * nrhs;
   }
+  // the solve's time includes every copy it derives from the matrix values
+  // (Jacobi inverse, block-sliced copy after a change the assembly or the
+  // penalty did not carry into it, FP32 copies)
   GF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
+  if (opts.precond == GF_PRECOND_JACOBI || opts.precision == GF_PREC_MIXED) {
+    st = prepare_jacobi(ctx, m);
+    if (st) return st;
+  }
+  const double* dinv = opts.precond == GF_PRECOND_JACOBI ? m->d_dinv : nullptr;
+  const int key = dinv ? 1 : 0;
+  if ((st = prepare_variant(ctx, m, P->variant))) return st;
+  PcgBuffers B = make_buffers(m, dinv);
   if (nrhs > 1 && opts.batched) {
     MultiHostState mh;
     std::vector<double> res(static_cast<size_t>(nrhs), 0.0);
-    int fallbacks_b = 0, refinements_b = 0;
-    bool done = false;
+    int fallbacks_b = 0, refinements_b = 0, mixed_it_b = 0;
+    bool done = false, warm_b = false;
     if (opts.precision == GF_PREC_MIXED) {
       MixedMultiHostState xh;
       st = run_multi_mixed(ctx, m, opts, Bd, Xd, nrhs, &xh, res.data());
       if (st) return st;
-      bool broke = false;
-      for (int k = 0; k < nrhs; ++k) broke |= xh.col_status[k] == GF_ERR_BREAKDOWN;
-      if (!broke) {
+      bool broke = false, stalled = false;
+      stalled = xh.ill != 0;
+      for (int k = 0; k < nrhs; ++k) {
+        broke |= xh.col_status[k] == GF_ERR_BREAKDOWN;
+        stalled |= xh.col_status[k] == kMixedStalled;
+      }
+      mixed_it_b = xh.it;
+      if (!broke && !stalled) {
         // the same per-column record as the FP64 path; the residual is the
         // FP64 one of the last refresh (the only one convergence is declared on)
         mh.it = xh.it;
@@ -489,17 +537,25 @@ int gf_pcg(gf_ctx* ctx, gf_ell* m, const double* b, double* x, int32_t nrhs,
         }
         done = true;
       } else {
-        // an FP32 operator that is not positive definite: zero-start FP64
-        // batched PCG for every column, as the single-RHS mixed solve does
+        // an FP32 operator that is not positive definite (breakdown):
+        // zero-start FP64 batched PCG for every column; an FP32 iteration
+        // that stalled: FP64 batched PCG from the current X (the columns
+        // that converged are done at its setup)
         fallbacks_b = 1;
+        warm_b = !broke;
       }
     }
     if (!done) {
       gf_solve_opts o64 = opts;
       o64.precision = GF_PREC_FP64;
-      if (fallbacks_b) o64.use_x0 = 0;
+      if (fallbacks_b) {
+        o64.use_x0 = warm_b ? 1 : 0;
+        o64.maxit = std::max(0, opts.maxit - mixed_it_b);
+      }
       st = run_multi(ctx, m, o64, Bd, Xd, nrhs, &mh, res.data());
       if (st) return st;
+      if (fallbacks_b)
+        for (int k = 0; k < nrhs; ++k) mh.col_it[k] += mixed_it_b;
     }
     GF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
     GF_HIP_CHECK(ctx, hipEventSynchronize(ctx->ev_stop));
@@ -545,16 +601,21 @@ int gf_pcg(gf_ctx* ctx, gf_ell* m, const double* b, double* x, int32_t nrhs,
       st = run_mixed(ctx, m, opts, &h, &refinements[size_t(k)]);
       if (st) return st;
       const bool ok = h.status == GF_OK && h.rr <= opts.rtol * opts.rtol * h.bnorm2;
-      if (!ok) {
-        // FP32 rounding can leave a badly conditioned operator indefinite:
-        // re-solve this right-hand side in FP64 from zero
-        const int mixed_it = h.it;
+      const int mixed_it = h.it;
+      if (!ok && opts.maxit - mixed_it > 0) {
+        // FP32 rounding can leave a badly conditioned operator indefinite
+        // (breakdown: re-solve in FP64 from zero) or unable to resolve the
+        // remaining error (stall: continue in FP64 from the current x), with
+        // the iterations the FP32 part left
+        const int warm = h.status == kMixedStalled ? 1 : 0;
         PcgBuffers Bj = make_buffers(m, m->d_dinv);
-        st = run_pcg(ctx, m, Bj, sched == GF_SCHED_PERSISTENT ? sched : GF_SCHED_KERNELS, 1, 0,
-                     opts.rtol, opts.maxit, opts.verify, &h);
+        st = run_pcg(ctx, m, Bj, sched == GF_SCHED_PERSISTENT ? sched : GF_SCHED_KERNELS, 1, warm,
+                     opts.rtol, opts.maxit - mixed_it, opts.verify, &h);
         if (st) return st;
         h.it += mixed_it;
         ++fallbacks;
+      } else if (!ok && h.status == kMixedStalled) {
+        h.status = GF_ERR_NOCONV;
       }
     } else {
       st = run_pcg(ctx, m, B, sched, key, opts.use_x0, opts.rtol, opts.maxit, opts.verify, &h);
@@ -593,6 +654,7 @@ int gf_pcg(gf_ctx* ctx, gf_ell* m, const double* b, double* x, int32_t nrhs,
   S.ms_device = ms;
   S.ms_host = now_ms() - h0;
   S.fallbacks = fallbacks;
+  for (int k = 0; k < nrhs; ++k) S.x0_rejected |= hs[size_t(k)].x0_rejected;
   if (stats) *stats = S;
   if (S.status == GF_ERR_BREAKDOWN)
     return set_error(ctx, GF_ERR_BREAKDOWN, "gf_pcg: breakdown (p.Kp <= 0 or NaN)");

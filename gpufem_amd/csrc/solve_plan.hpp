@@ -32,6 +32,9 @@ namespace gf {
 void release_solve_plan(gf_ell* m);
 int ensure_plan(gf_ctx* ctx, gf_ell* m);
 void set_variant(gf_ell* m, int variant);
+// workgroups of the PCG kernels for this variant and size (the grid every
+// schedule uses, so all give bitwise-identical results)
+int pcg_grid(int variant, int64_t n, int num_cu);
 int tune_spmv(gf_ctx* ctx, gf_ell* m, int force, double* ms_out, int* chosen);
 PcgBuffers make_buffers(gf_ell* m, const double* dinv);
 // arrays the SpMV variant reads: the ELL arrays, the node columns (kNode) or
@@ -42,6 +45,8 @@ struct SpmvOperands {
   BsellView bs;
 };
 SpmvOperands spmv_operands(const gf_ell* m, int variant);
+// the block-sliced copy of the matrix (valid after prepare_variant of a kBsell variant)
+BsellView bsell_view(const gf_ell* m);
 // builds what the variant reads and refreshes copies of stale values
 int prepare_variant(gf_ctx* ctx, gf_ell* m, int variant);
 // the node-block / block-sliced (swept) variant of the matrix's dimension, or -1

@@ -15,8 +15,12 @@
 // sequential sweep makes, so the coloring equals the host phase's (and the
 // reference's); each round colors at least the lowest uncolored element.
 // Colors are kept as bits of a 64-bit mask (at most 64 colors).
-#include <hipcub/hipcub.hpp>
-
+//
+// Prefix sums (incidence and adjacency pointers) and the grouping of the
+// elements by color (femColor.cc:172-179 appends each element to its color's
+// list in element order) are done here too: a three-pass tiled exclusive
+// scan, and a stable counting sort over tiles of elements whose per-(color,
+// tile) counts are scanned color-major.  Both are deterministic.
 #include <algorithm>
 #include <string>
 #include <vector>
@@ -200,10 +204,119 @@ __global__ void k_color_round(const int32_t* __restrict__ conn, int64_t nelem, i
   if (any && (threadIdx.x & 63) == 0) *pending = 1;
 }
 
-__global__ void k_iota(int32_t* p, int64_t n) {
-  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
-       t += int64_t(gridDim.x) * blockDim.x)
-    p[t] = int32_t(t);
+// ---- exclusive prefix sums of int32 counts into int64 -------------------------
+// Tiles of kScanItems consecutive entries, kScanThreads threads of
+// kScanItems / kScanThreads consecutive entries each.
+constexpr int kScanThreads = 256;
+constexpr int kScanPer = 8;
+constexpr int kScanItems = kScanThreads * kScanPer;
+
+// inclusive scan of one value per thread over the workgroup (LDS)
+__device__ inline int64_t block_inclusive_scan(int64_t v, int64_t* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    const int64_t add = t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += add;
+    __syncthreads();
+  }
+  const int64_t r = sh[t];
+  __syncthreads();
+  return r;
+}
+
+// tile_sum[b] = sum of tile b
+__global__ __launch_bounds__(kScanThreads) void k_scan_tile_sums(const int32_t* __restrict__ in,
+                                                                 int64_t n,
+                                                                 int64_t* __restrict__ tile_sum) {
+  __shared__ int64_t sh[kScanThreads];
+  const int64_t base = int64_t(blockIdx.x) * kScanItems + int64_t(threadIdx.x) * kScanPer;
+  int64_t v = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k)
+    if (base + k < n) v += in[base + k];
+  v = block_inclusive_scan(v, sh);
+  if (threadIdx.x == kScanThreads - 1) tile_sum[blockIdx.x] = v;
+}
+
+// one workgroup: exclusive scan of the ntiles tile sums in place
+__global__ __launch_bounds__(kScanThreads) void k_scan_tile_offsets(int64_t* tile_sum,
+                                                                    int64_t ntiles) {
+  __shared__ int64_t sh[kScanThreads];
+  int64_t carry = 0;
+  for (int64_t c0 = 0; c0 < ntiles; c0 += kScanThreads) {
+    const int64_t i = c0 + threadIdx.x;
+    const int64_t v = i < ntiles ? tile_sum[i] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+      const int64_t add = int(threadIdx.x) >= off ? sh[threadIdx.x - off] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += add;
+      __syncthreads();
+    }
+    if (i < ntiles) tile_sum[i] = carry + sh[threadIdx.x] - v;
+    carry += sh[kScanThreads - 1];
+    __syncthreads();
+  }
+}
+
+// out[i] = tile offset + exclusive sum inside the tile
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const int32_t* __restrict__ in,
+                                                             int64_t n,
+                                                             const int64_t* __restrict__ tile_off,
+                                                             int64_t* __restrict__ out) {
+  __shared__ int64_t sh[kScanThreads];
+  const int64_t base = int64_t(blockIdx.x) * kScanItems + int64_t(threadIdx.x) * kScanPer;
+  int32_t v[kScanPer];
+  int64_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    v[k] = base + k < n ? in[base + k] : 0;
+    sum += v[k];
+  }
+  int64_t run = tile_off[blockIdx.x] + block_inclusive_scan(sum, sh) - sum;
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+}
+
+// ---- elements grouped by color, ascending element id inside a color ---------
+// One thread walks a tile of kColorTile consecutive elements; 64 threads per
+// workgroup keep their 64 per-color counters in LDS ([color][thread]).
+constexpr int kColorTile = 256;
+constexpr int kColorThreads = 64;
+
+// cnt[(c - 1) * ntiles + t] = elements of color c in tile t (colors 1..64)
+__global__ __launch_bounds__(kColorThreads) void k_color_tile_counts(
+    const int32_t* __restrict__ color, int64_t nelem, int64_t ntiles, int32_t* __restrict__ cnt) {
+  __shared__ int32_t c64[64][kColorThreads];
+  const int tid = threadIdx.x;
+  const int64_t t = int64_t(blockIdx.x) * kColorThreads + tid;
+  for (int c = 0; c < 64; ++c) c64[c][tid] = 0;
+  if (t < ntiles) {
+    const int64_t e1 = std::min<int64_t>(nelem, (t + 1) * kColorTile);
+    for (int64_t e = t * kColorTile; e < e1; ++e) c64[color[e] - 1][tid]++;
+    for (int c = 0; c < 64; ++c) cnt[int64_t(c) * ntiles + t] = c64[c][tid];
+  }
+}
+
+// each tile writes its elements in element order from its color-major
+// offsets (off = exclusive sums of cnt)
+__global__ __launch_bounds__(kColorThreads) void k_color_tile_scatter(
+    const int32_t* __restrict__ color, int64_t nelem, int64_t ntiles,
+    const int64_t* __restrict__ off, int32_t* __restrict__ out) {
+  __shared__ int64_t pos[64][kColorThreads];
+  const int tid = threadIdx.x;
+  const int64_t t = int64_t(blockIdx.x) * kColorThreads + tid;
+  if (t >= ntiles) return;
+  for (int c = 0; c < 64; ++c) pos[c][tid] = off[int64_t(c) * ntiles + t];
+  const int64_t e1 = std::min<int64_t>(nelem, (t + 1) * kColorTile);
+  for (int64_t e = t * kColorTile; e < e1; ++e) out[pos[color[e] - 1][tid]++] = int32_t(e);
 }
 
 // device allocations released on every exit path
@@ -249,14 +362,19 @@ int build_symbolic_device(hipStream_t s, const int32_t* d_conn, int64_t nelem, i
   hipLaunchKernelGGL(k_incidence_count, dim3(grid_for(total, 256, 16384)), dim3(256), 0, s,
                      d_conn, total, nnodes, d_cnt, d_flags);
   TRY(hipGetLastError());
-  // exclusive sums over nnodes + 1 entries (the last count is 0 -> the total)
+  // exclusive sums (over nnodes + 1 entries, the last count 0: the total)
   auto exclusive_sum = [&](const int32_t* in, int64_t* outp, int64_t n) -> hipError_t {
-    size_t bytes = 0;
-    hipError_t r = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, outp, int(n), s);
+    if (n <= 0) return hipSuccess;
+    const int64_t ntiles = (n + kScanItems - 1) / kScanItems;
+    int64_t* tile_sum = nullptr;
+    hipError_t r = tmp.alloc(&tile_sum, size_t(ntiles));
     if (r != hipSuccess) return r;
-    void* ws = nullptr;
-    if ((r = tmp.alloc(reinterpret_cast<char**>(&ws), bytes)) != hipSuccess) return r;
-    return hipcub::DeviceScan::ExclusiveSum(ws, bytes, in, outp, int(n), s);
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(unsigned(ntiles)), dim3(kScanThreads), 0, s, in, n,
+                       tile_sum);
+    hipLaunchKernelGGL(k_scan_tile_offsets, dim3(1), dim3(kScanThreads), 0, s, tile_sum, ntiles);
+    hipLaunchKernelGGL(k_scan_apply, dim3(unsigned(ntiles)), dim3(kScanThreads), 0, s, in, n,
+                       tile_sum, outp);
+    return hipGetLastError();
   };
   TRY(exclusive_sum(d_cnt, d_n2e_ptr, nnodes + 1));
   int32_t h_flags[4] = {0, 0, 0, 0};
@@ -336,19 +454,21 @@ int build_symbolic_device(hipStream_t s, const int32_t* d_conn, int64_t nelem, i
   if (nelem > 0)
     TRY(hipMemcpyAsync(out->elem_color.data(), d_color, size_t(nelem) * sizeof(int32_t),
                        hipMemcpyDeviceToHost, s));
-  int32_t *d_ids = nullptr, *d_keys_out = nullptr;
-  TRY(tmp.alloc(&d_ids, size_t(nelem)));
-  TRY(tmp.alloc(&d_keys_out, size_t(nelem)));
   TRY(hipMalloc(&out->d_color_elems, size_t(nelem > 0 ? nelem : 1) * sizeof(int32_t)));
   if (nelem > 0) {
-    hipLaunchKernelGGL(k_iota, dim3(gel), dim3(256), 0, s, d_ids, nelem);
-    size_t bytes = 0;
-    TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, d_color, d_keys_out, d_ids,
-                                           out->d_color_elems, int(nelem), 0, 7, s));
-    char* ws = nullptr;
-    TRY(tmp.alloc(&ws, bytes));
-    TRY(hipcub::DeviceRadixSort::SortPairs(ws, bytes, d_color, d_keys_out, d_ids,
-                                           out->d_color_elems, int(nelem), 0, 7, s));
+    const int64_t ntiles = (nelem + kColorTile - 1) / kColorTile;
+    int32_t* d_tcnt = nullptr;
+    int64_t* d_toff = nullptr;
+    TRY(tmp.alloc(&d_tcnt, size_t(64 * ntiles)));
+    TRY(tmp.alloc(&d_toff, size_t(64 * ntiles)));
+    const unsigned gt = unsigned((ntiles + kColorThreads - 1) / kColorThreads);
+    hipLaunchKernelGGL(k_color_tile_counts, dim3(gt), dim3(kColorThreads), 0, s, d_color, nelem,
+                       ntiles, d_tcnt);
+    TRY(hipGetLastError());
+    TRY(exclusive_sum(d_tcnt, d_toff, 64 * ntiles));
+    hipLaunchKernelGGL(k_color_tile_scatter, dim3(gt), dim3(kColorThreads), 0, s, d_color, nelem,
+                       ntiles, d_toff, out->d_color_elems);
+    TRY(hipGetLastError());
   }
   TRY(hipStreamSynchronize(s));
 #undef TRY

@@ -303,7 +303,8 @@ def evaluate(ctx, fam, net, c_u, ks, rtol=1e-8, log=log_default):
                     base_channel=BASE_CHANNEL, base_weight=fam.base_weight(c_u))
         x0 = d_x.download(np.float64)  # also synchronises
         t_inf = time.perf_counter() - t0
-        st1 = _pcg_dev(ctx, K, d_b, d_x, 1, rtol)
+        # the guarded guess: kept only when its residual is below ||b||
+        st1 = _pcg_dev(ctx, K, d_b, d_x, 2, rtol)
         u1 = d_x.download(np.float64)
         rec = {
             "k": int(k),
@@ -311,6 +312,7 @@ def evaluate(ctx, fam, net, c_u, ks, rtol=1e-8, log=log_default):
             "zero_true_relres": float(st0.true_relres),
             "warm_it": int(st1.iterations), "warm_ms": float(st1.ms_device),
             "warm_true_relres": float(st1.true_relres),
+            "warm_x0_rejected": int(st1.x0_rejected),
             "inference_ms": 1e3 * t_inf,
             "x0_rel_err": float(np.linalg.norm(x0 - u0) / np.linalg.norm(u0)),
             # the untrained network's guess (the beam estimate) for comparison
@@ -349,6 +351,7 @@ def summarize(recs):
         "x0_rel_err_mean": float(np.mean([r["x0_rel_err"] for r in recs])),
         "base_rel_err_mean": float(np.mean([r.get("base_rel_err", np.nan) for r in recs])),
         "max_solution_rel_diff": float(max(r["solution_rel_diff"] for r in recs)),
+        "warm_x0_rejected": int(sum(r.get("warm_x0_rejected", 0) for r in recs)),
     }
 
 

@@ -132,6 +132,20 @@ def config_brick20_cube(n=48):
     return structured(GF_BRICK20, n, n, n, ngauss=3, name="brick20_cube_%d" % n)
 
 
+def by_name(name):
+    """The benchmark workload of that name (q4_plate_NXxNY, q8_plate_NXxNY,
+    brick8_cube_N, brick20_cube_N)."""
+    for prefix, make in (("q4_plate_", config_q4_plate), ("q8_plate_", config_q8_plate)):
+        if name.startswith(prefix):
+            nx, ny = map(int, name[len(prefix):].split("x"))
+            return make(nx, ny)
+    if name.startswith("brick8_cube_"):
+        return config_brick8_cube(int(name[len("brick8_cube_"):]))
+    if name.startswith("brick20_cube_"):
+        return config_brick20_cube(int(name[len("brick20_cube_"):]))
+    raise ValueError("unknown workload %s" % name)
+
+
 def q4_batch_sizes(count=256, lo=16, hi=256, seed=SEED_Q4_BATCH):
     nx = rng_uniform_int(seed, count, lo, hi)
     return [(int(a), int(a) // 2) for a in nx]

@@ -176,9 +176,67 @@ def sweep(ctx, paths, out, gauss=None, solve=True, append=False, reference_solve
     return recs
 
 
+# The meshes of the reference's published assembly tables
+# (Report/test_models_home/*/_*_col.txt) and the table each appears in.
+REFERENCE_MESHES = (("placa128_Q4.nf", "Q4_col.txt"), ("placa128_Q8.nf", "Q8_col.txt"),
+                    ("rubik18_brk8.nf", "brk8_col.txt"), ("rubik20_brk20.nf", "brk20_col.txt"))
+
+
+def reference_asm_times(table, dof, gauss):
+    """{threads: (AsmCPU, AsmGPU)} of the ELL-format rows of a reference
+    result table for the mesh with `dof` DOF and `gauss` points per axis."""
+    out = {}
+    for blk in read_results(table):
+        if blk["format"] != "ELL" or blk["gauss"] != gauss or "AsmGPU" not in blk["columns"]:
+            continue
+        cols = blk["columns"]
+        for row in blk["rows"]:
+            if row and int(row[0]) == dof:
+                out[blk["threads"]] = (row[cols.index("AsmCPU")], row[cols.index("AsmGPU")])
+    return out
+
+
+def reference_meshes(ctx, nf_dir, table_dir, log=None):
+    """Assembles the reference's benchmark meshes with 2 and 3 Gauss points
+    per axis and returns one record per (mesh, rule) with this library's
+    numeric and full (symbolic + numeric) assembly times beside the
+    reference's own AsmCPU / AsmGPU seconds at 1..4 threads.  The
+    reference's AsmGPU covers allocation, copies, the OpenCL Ke kernel, the
+    read-back and the host scatter (src/FEM/StiffAlgoGpuOmp.cc:86-172)."""
+    from . import nf
+    recs = []
+    for name, table in REFERENCE_MESHES:
+        for gauss in (2, 3):
+            p = nf.read_nf(os.path.join(nf_dir, name), ngauss=gauss)
+            mesh = ctx.upload_mesh(p.elem_type, p.ngauss, p.conn, p.coords, p.E, p.nu)
+            t0 = time.perf_counter()
+            K = ctx.symbolic(mesh)
+            t_sym = time.perf_counter() - t0
+            variant, ms = ctx.assemble_tune(mesh, K)
+            t = min(ctx.assemble(mesh, K).ms_device for _ in range(5)) / 1e3
+            ref = reference_asm_times(os.path.join(table_dir, table), p.ndof, gauss)
+            best_gpu = min((v[1] for v in ref.values()), default=None)
+            rec = {"mesh": name, "dof": int(p.ndof), "elements": int(p.conn.shape[0]),
+                   "gauss": gauss, "assembly_variant": int(variant), "asm_s": t,
+                   "asm_dof_per_s": p.ndof / t, "symbolic_s": t_sym,
+                   "full_dof_per_s": p.ndof / (t + t_sym),
+                   "reference": {str(k): {"AsmCPU": v[0], "AsmGPU": v[1]} for k, v in ref.items()},
+                   "reference_best_AsmGPU_s": best_gpu,
+                   "speedup_vs_reference_AsmGPU": (best_gpu / t) if best_gpu else None}
+            recs.append(rec)
+            if log:
+                log("%s %d GP: %d DOF, assembly %.4f ms (%.3g DOF/s), full %.3f s; "
+                    "reference AsmGPU best %s s" % (name, gauss, p.ndof, t * 1e3, p.ndof / t,
+                                                    t + t_sym, best_gpu))
+            K.free()
+            mesh.free()
+    return recs
+
+
 def main(argv=None):
     """python -m gpufem_amd.results OUT file.nf ... [--append] [--asm-only]
-    [--pcg] [--gauss N]"""
+    [--pcg] [--gauss N]
+    python -m gpufem_amd.results --reference-meshes NF_DIR TABLE_DIR"""
     import argparse
     import json
     from . import Context
@@ -190,6 +248,15 @@ def main(argv=None):
     ap.add_argument("--pcg", action="store_true",
                     help="solver columns: Jacobi-PCG to 1e-8 instead of the reference's CG")
     ap.add_argument("--gauss", type=int, default=None)
+    if argv is None:
+        import sys
+        argv = sys.argv[1:]
+    if argv and argv[0] == "--reference-meshes":
+        ctx = Context(0)
+        recs = reference_meshes(ctx, argv[1], argv[2],
+                                log=lambda m: print("[results] " + m, flush=True))
+        print(json.dumps(recs))
+        return
     a = ap.parse_args(argv)
     ctx = Context(0)
     recs = sweep(ctx, a.files, a.out, gauss=a.gauss, solve=not a.asm_only, append=a.append,

@@ -250,7 +250,10 @@ typedef struct gf_solve_opts {
   int32_t precond;   /* GF_PRECOND_* */
   int32_t precision; /* GF_PREC_* */
   int32_t schedule;  /* GF_SCHED_* */
-  int32_t use_x0;    /* 1: x holds the initial guess; 0: start from zero */
+  int32_t use_x0;    /* 1: x holds the initial guess; 0: start from zero; 2 (FP64,
+                        single right-hand side; elsewhere as 1): x holds a guess that
+                        is kept only when ||b - K x|| < ||b||, else the solve starts
+                        from zero (zero start is the fallback) */
   int32_t maxit;     /* iteration cap (per right-hand side) */
   int32_t device_ptrs; /* 1: b and x are device pointers */
   double rtol;       /* stop when ||b - K x||_2 <= rtol ||b||_2 */
@@ -274,9 +277,13 @@ typedef struct gf_solve_stats {
   double true_relres;   /* max over RHS of ||b - K x||/||b|| recomputed in FP64 */
   double ms_device;     /* event time of the solve on the stream */
   double ms_host;       /* host wall time of the call */
-  int32_t fallbacks;    /* right-hand sides re-solved by zero-start FP64 PCG after
-                           the mixed-precision iteration broke down or stalled */
-  int32_t pad_;
+  int32_t fallbacks;    /* GF_PREC_MIXED: right-hand sides (batched: solves) finished by
+                           FP64 PCG: from zero after the FP32 iteration broke down, from
+                           the current x once the FP32 operator proved too ill-conditioned
+                           (condition estimate from the CG coefficients above 1/u32) or
+                           the refreshes stopped halving the true residual */
+  int32_t x0_rejected;  /* 1 if a guarded initial guess (use_x0 = 2) was replaced by
+                           zero */
 } gf_solve_stats;
 
 void gf_solve_opts_default(gf_solve_opts* o);

@@ -15,7 +15,10 @@ HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "nf")
 
 
 @pytest.mark.parametrize("name", ["rubik10_brk8.nf", "brick_brk20v2.nf", "placa2_Q4.nf",
-                                  "placa2_Q8.nf", "brick_brk8v2.nf"])
+                                  "placa2_Q8.nf", "brick_brk8v2.nf",
+                                  # the meshes of the reference's published tables
+                                  "placa128_Q4.nf", "placa128_Q8.nf", "rubik18_brk8.nf",
+                                  "rubik20_brk20.nf"])
 def test_reference_mesh_matches_oracle(gpu_ctx, name):
     p = nf.read_nf(os.path.join(HERE, name))
     mesh = gpu_ctx.upload_mesh(p.elem_type, p.ngauss, p.conn, p.coords, p.E, p.nu)
@@ -37,3 +40,14 @@ def test_reference_mesh_matches_oracle(gpu_ctx, name):
     xo, ito, _, _ = O.pcg_jacobi(Ko, p.f, rtol=1e-8)
     assert st.converged and st.true_relres <= 1e-8
     assert np.linalg.norm(x - xo) <= 1e-9 * np.linalg.norm(xo)
+
+
+def test_reference_mesh_sweep_reads_reference_tables(gpu_ctx):
+    """The reference-mesh sweep assembles each mesh with both Gauss rules and
+    finds the reference's own AsmGPU rows for it (1..4 threads)."""
+    from gpufem_amd import results
+    recs = results.reference_meshes(gpu_ctx, HERE, os.path.join(os.path.dirname(HERE), "results"))
+    assert len(recs) == 8
+    for r in recs:
+        assert set(r["reference"]) == {"1", "2", "3", "4"}, (r["mesh"], r["gauss"])
+        assert r["asm_s"] > 0 and r["reference_best_AsmGPU_s"] > 0
