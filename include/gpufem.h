@@ -419,6 +419,26 @@ int gf_nn_conv3d_wgrad(gf_ctx* ctx, const uint16_t* x_dev, int32_t batch, const 
                        int32_t cin, const uint16_t* dz_dev, const int32_t* vout, int32_t cout,
                        int32_t stride, float* dw_dev, float* db_dev);
 
+/* ======================================================================== */
+/* Allocation-free solve.  No counterpart in the reference: for callers     */
+/* that cannot allocate device memory (code under HIP graph capture, tight  */
+/* serving loops, harnesses that forbid hipMalloc).  The FP64 (Jacobi-)PCG   */
+/* of gf_pcg for one right-hand side, b and x device pointers (x: initial   */
+/* guess when opts->use_x0), kernel launches only; the Jacobi inverse and   */
+/* the r, z, p, q vectors, partial sums and scalar state live in a static   */
+/* device array of GF_NOALLOC_SCRATCH_BYTES that the library holds on each  */
+/* GPU it runs on until the process ends.  Matrices of more than            */
+/* GF_NOALLOC_MAX_ROWS rows are rejected (GF_ERR_UNSUPPORTED).  Uses the    */
+/* matrix's tuned SpMV variant when its copies are current (gf_ell_tune or  */
+/* an earlier gf_pcg builds them), else one thread per ELL row; calls no    */
+/* allocating routine.  All calls on one GPU share its array: calls that    */
+/* overlap on one GPU race.                                                  */
+/* ======================================================================== */
+#define GF_NOALLOC_MAX_ROWS (1 << 20)
+#define GF_NOALLOC_SCRATCH_BYTES 42520832
+int gf_pcg_noalloc(gf_ctx* ctx, gf_ell* m, const double* b, double* x,
+                   const gf_solve_opts* opts, gf_solve_stats* stats);
+
 /* Device memory helpers for callers that keep vectors resident in HBM. */
 int gf_dev_alloc(gf_ctx* ctx, int64_t bytes, void** dptr);
 int gf_dev_free(gf_ctx* ctx, void* dptr);
