@@ -25,7 +25,7 @@ GF_PRECOND_NONE, GF_PRECOND_JACOBI = 0, 1
 GF_PREC_FP64, GF_PREC_MIXED = 0, 1
 GF_SCHED_AUTO, GF_SCHED_KERNELS, GF_SCHED_GRAPH, GF_SCHED_PERSISTENT = 0, 1, 2, 3
 GF_SPMV_VARIANTS = 11
-GF_ASM_VARIANTS = 4  # entries gf_assemble_tune writes into ms_per_variant
+GF_ASM_VARIANTS = 5  # entries gf_assemble_tune writes into ms_per_variant
 
 
 class GridDesc(C.Structure):
@@ -125,6 +125,7 @@ SIGNATURES = {
     "gf_nn_conv3d": (C.c_int, [_P, _P, _I32, _P, _I32, _P, _P, _I32, _I32, _I32, _I32,
                                C.c_float, _P, _P, _P, _I32]),
     "gf_nn_conv3d_wgrad": (C.c_int, [_P, _P, _I32, _P, _I32, _P, _P, _I32, _I32, _P, _P]),
+    "gf_pcg_noalloc": (C.c_int, [_P, _P, _P, _P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "gf_dev_alloc": (C.c_int, [_P, _I64, C.POINTER(_P)]),
     "gf_dev_free": (C.c_int, [_P, _P]),
     "gf_copy_h2d": (C.c_int, [_P, _P, _P, _I64]),

@@ -232,6 +232,7 @@ static void ell_release(gf_ell* m) {
   dfree(m->d_slot);
   dfree(m->d_n2e_ptr);
   dfree(m->d_n2e);
+  dfree(m->d_egrad);
   dfree(m->d_val32);
   dfree(m->d_dinv);
   release_solve_plan(m);
@@ -496,19 +497,30 @@ static bool gather_fits(const gf_mesh* mesh, const gf_ell* m) {
   return cap > 0 && m->width / m->dim <= cap && mesh->nelem < (int64_t(1) << 26);
 }
 
+// the two-phase kernel: every node fits its slots
+static bool pairs_fits(const gf_mesh* mesh, const gf_ell* m) {
+  const int cap = pairs_max_slots(mesh->type);
+  return cap > 0 && m->width / m->dim <= cap && mesh->nelem < (int64_t(1) << 26);
+}
+
 static int assemble_with(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int variant, gf_timing* t) {
   const double h0 = now_ms();
-  if (variant == 3) {
+  if (variant == 3 || variant == 4) {
     int s3 = ensure_n2e(ctx, mesh, m);
     if (s3) return s3;
+  }
+  if (variant == 4 && !m->d_egrad) {
+    const int64_t cnt = pairs_egrad_doubles(mesh->type, mesh->ngauss, mesh->nelem);
+    GF_HIP_CHECK(ctx, hipMalloc(&m->d_egrad, size_t(std::max<int64_t>(cnt, 1)) * sizeof(double)));
   }
   int* d_bad = reinterpret_cast<int*>(static_cast<char*>(ctx->d_state) + 128);
   GF_HIP_CHECK(ctx, hipMemsetAsync(d_bad, 0, sizeof(int), ctx->stream));
   int st = record_start(ctx);
   if (st) return st;
   const size_t slots = size_t(m->n) * size_t(m->width);
-  // the gather writes every live slot; the scatters accumulate from zero
-  if (variant != 3)
+  // the gather and the two-phase kernel write every live slot; the scatters
+  // accumulate from zero
+  if (variant != 3 && variant != 4)
     GF_HIP_CHECK(ctx, hipMemsetAsync(m->d_val, 0, slots * sizeof(double), ctx->stream));
   AssemblyArgs a{};
   a.type = mesh->type;
@@ -531,13 +543,18 @@ static int assemble_with(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int varian
   // inside the assembly (and its time): the node gather writes it beside
   // the ELL values, the colored scatters are followed by the copy
   const bool bsell = m->d_bval && m->bval_version != 0;
-  a.bval = bsell && variant == 3 ? m->d_bval : nullptr;
+  a.bval = bsell && (variant == 3 || variant == 4) ? m->d_bval : nullptr;
   a.bsptr = m->d_bsptr;
   a.binv = m->d_binv;
-  int64_t launches = variant == 3 ? 0 : 1;
+  a.col = m->d_col;
+  a.nelem = mesh->nelem;
+  int64_t launches = variant == 3 || variant == 4 ? 0 : 1;
   if (variant == 3) {
     GF_HIP_CHECK(ctx, launch_assembly_gather(ctx->stream, a));
     ++launches;
+  } else if (variant == 4) {
+    GF_HIP_CHECK(ctx, launch_assembly_pairs(ctx->stream, a, m->d_egrad));
+    launches += 2;
   } else {
     for (int c = 0; c < m->ncolors; ++c) {
       a.elems = m->d_color_elems + m->color_ptr[size_t(c)];
@@ -546,7 +563,7 @@ static int assemble_with(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int varian
       ++launches;
     }
   }
-  if (bsell && variant != 3) {
+  if (bsell && variant != 3 && variant != 4) {
     GF_HIP_CHECK(ctx, launch_bsell_vals(ctx->stream, bsell_view(m), m->dim, m->n, m->d_val,
                                         m->d_rownnz, m->d_bval));
     ++launches;
@@ -581,15 +598,23 @@ int gf_assemble_tune(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int32_t force,
                      double* ms_per_variant, int32_t* chosen) {
   int st = check_assembly_args(ctx, mesh, m, "gf_assemble_tune");
   if (st) return st;
-  if (force > 3 || (force == 2 && mesh->dim != 3))
-    return set_error(ctx, GF_ERR_ARG, "gf_assemble_tune: variant must be 0, 1, 3 or (3D) 2");
+  if (force > 4 || (force == 2 && mesh->dim != 3))
+    return set_error(ctx, GF_ERR_ARG, "gf_assemble_tune: variant must be 0, 1, 3, 4 or (3D) 2");
   if (force == 3 && !gather_fits(mesh, m))
     return set_error(ctx, GF_ERR_UNSUPPORTED,
                      "gf_assemble_tune: the node-gather kernel does not fit this mesh");
+  if (force == 4 && !pairs_fits(mesh, m))
+    return set_error(ctx, GF_ERR_UNSUPPORTED,
+                     "gf_assemble_tune: the two-phase kernel does not fit this mesh");
   if (ms_per_variant)
     for (int v = 0; v < GF_ASM_VARIANTS; ++v) ms_per_variant[v] = -1.0;
   if (force >= 0) {
     m->asm_variant = force;
+    if (force != 4 && m->d_egrad) {  // the two-phase kernel's buffer is no longer needed
+      GF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+      GF_HIP_CHECK(ctx, hipFree(m->d_egrad));
+      m->d_egrad = nullptr;
+    }
   } else {
     double best = 1e300;
     int best_v = 0;

@@ -99,6 +99,8 @@ struct gf_ell {
   // assembly; lazily built)
   int64_t* d_n2e_ptr = nullptr;
   int32_t* d_n2e = nullptr;
+  // element gradients at the Gauss points (two-phase assembly; lazily built)
+  double* d_egrad = nullptr;
   std::vector<int32_t> h_elem_color;  // [nelem], 1-based colors
   int64_t nelem = 0;
   int nen = 0;

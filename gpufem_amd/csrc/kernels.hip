@@ -874,6 +874,159 @@ hipError_t launch_assembly_color(hipStream_t s, const AssemblyArgs& a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Two-phase assembly (variant 4).
+//
+// Phase 1, one thread per (element, Gauss point): J, det J, J^-1 and the
+// physical gradients of the element's nen shape functions, G[a][i] =
+// sum_k Jinv[i][k] dN_a/dxi_k, and the scale w_g det J E_e, written once to
+// egrad[(e ngp + g) kGS ...] (kGS = nen dim + 1 doubles).
+//
+// Phase 2, one thread per (node v, neighbour slot s), node-major: the d x d
+// block K_vw (w = the slot's node) summed over the elements that contain
+// both nodes (in ascending element order) and their Gauss points,
+//   K_vw[i][j] = sum_e sum_g sc (G_a . D . G_b)[i][j]
+// with the isotropic / plane-stress D written out (no B zeros multiplied),
+// kept in registers and written once into the ELL values (and the
+// block-sliced copy when it exists).  Each element's geometry is computed
+// once (the node gather recomputes it for every row of each of its nodes),
+// no coloring, no read-modify-write, and no LDS: the occupancy is set by
+// registers alone.
+// ---------------------------------------------------------------------------
+struct PArgs {
+  KArgs k;
+  double* egrad;
+  int64_t nelem;
+  const int64_t* n2e_ptr;
+  const int32_t* n2e;
+  const int32_t* rownnz;
+  const int32_t* col;
+  int64_t nnodes;
+  double* bval;
+  const int64_t* bsptr;
+  const int32_t* binv;
+};
+
+template <int DIM, int NEN, int NGP>
+__global__ __launch_bounds__(256) void k_elem_gradients(PArgs pa) {
+  constexpr int GS = NEN * DIM + 1;
+  const KArgs& a = pa.k;
+  const int64_t units = pa.nelem * NGP;
+  for (int64_t u = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; u < units;
+       u += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t e = u / NGP;
+    const int g = int(u - e * NGP);
+    double x[NEN][DIM];
+#pragma unroll
+    for (int n = 0; n < NEN; ++n) {
+      const int64_t node = a.conn[e * NEN + n] - 1;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) x[n][d] = a.coords[node * DIM + d];
+    }
+    const double* dN = c_tab + a.tab + g * DIM * NEN;
+    double J[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) {
+        double acc = 0.0;
+#pragma unroll
+        for (int n = 0; n < NEN; ++n) acc += dN[i * NEN + n] * x[n][j];
+        J[i][j] = acc;
+      }
+    double Ji[DIM][DIM];
+    const double det = det_inv<DIM>(J, Ji);
+    if (!(det > 0.0)) *a.bad = 1;
+    const double Ee = a.elemE ? a.elemE[e] : a.E;
+    double* out = pa.egrad + u * GS;
+#pragma unroll
+    for (int n = 0; n < NEN; ++n)
+#pragma unroll
+      for (int i = 0; i < DIM; ++i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) acc += Ji[i][k] * dN[k * NEN + n];
+        out[n * DIM + i] = acc;
+      }
+    out[NEN * DIM] = c_tab[a.tab + NGP * DIM * NEN + g] * det * Ee;
+  }
+}
+
+template <int DIM, int NEN, int NGP, int MAXS>
+__global__ __launch_bounds__(256) void k_assemble_pairs(PArgs pa) {
+  constexpr int GS = NEN * DIM + 1;
+  const KArgs& a = pa.k;
+  const int64_t units = pa.nnodes * MAXS;
+  for (int64_t u = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; u < units;
+       u += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t v = u / MAXS;
+    const int s = int(u - v * MAXS);
+    const int64_t row0 = v * DIM;
+    if (s >= pa.rownnz[row0] / DIM) continue;
+    const int32_t w = pa.col[row0 + int64_t(s) * DIM * a.n] / DIM;
+    double acc[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) acc[i][j] = 0.0;
+    const int64_t k1 = pa.n2e_ptr[v + 1];
+    for (int64_t kk = pa.n2e_ptr[v]; kk < k1; ++kk) {
+      const int32_t packed = pa.n2e[kk];
+      const int64_t e = packed >> 5;
+      const int la = packed & 31;
+      int lb = -1;
+#pragma unroll
+      for (int q = 0; q < NEN; ++q)
+        if (a.conn[e * NEN + q] - 1 == w) lb = q;
+      if (lb < 0) continue;
+      const double* ge = pa.egrad + e * NGP * GS;
+      for (int g = 0; g < NGP; ++g) {
+        const double* gp = ge + g * GS;
+        double gA[DIM], gB[DIM];
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+          gA[i] = gp[la * DIM + i];
+          gB[i] = gp[lb * DIM + i];
+        }
+        const double sc = gp[NEN * DIM];
+        if constexpr (DIM == 2) {
+          // plane stress: K_ii = c0 gA_i gB_i + c2 gA_k gB_k, K_ij = c1 gA_i gB_j + c2 gA_j gB_i
+          const double t00 = sc * gA[0] * gB[0], t11 = sc * gA[1] * gB[1];
+          const double t01 = sc * gA[0] * gB[1], t10 = sc * gA[1] * gB[0];
+          acc[0][0] += a.c0 * t00 + a.c2 * t11;
+          acc[1][1] += a.c0 * t11 + a.c2 * t00;
+          acc[0][1] += a.c1 * t01 + a.c2 * t10;
+          acc[1][0] += a.c1 * t10 + a.c2 * t01;
+        } else {
+          const double dot = gA[0] * gB[0] + gA[1] * gB[1] + gA[2] * gB[2];
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+              const double kij = i == j ? a.c0 * gA[i] * gB[i] + a.c2 * (dot - gA[i] * gB[i])
+                                        : a.c1 * gA[i] * gB[j] + a.c2 * gA[j] * gB[i];
+              acc[i][j] += sc * kij;
+            }
+        }
+      }
+    }
+    int64_t bb = 0;
+    int bl = 0;
+    if (pa.bval) {
+      const int32_t pos = pa.binv[v];
+      bb = pa.bsptr[pos >> 6] + s;
+      bl = pos & 63;
+    }
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) {
+        a.val[row0 + i + (int64_t(s) * DIM + j) * a.n] = acc[i][j];
+        if (pa.bval) pa.bval[bsell_vidx(DIM * DIM, bb, i * DIM + j, bl)] = acc[i][j];
+      }
+  }
+}
+
 namespace {
 template <int DIM, int NEN, int NGP, int MAXS>
 hipError_t launch_gather(hipStream_t s, const GArgs& g) {
@@ -888,6 +1041,81 @@ hipError_t launch_gather(hipStream_t s, const GArgs& g) {
   return hipGetLastError();
 }
 }  // namespace
+
+namespace {
+template <int DIM, int NEN, int NGP, int MAXS>
+hipError_t launch_pairs(hipStream_t s, const PArgs& p) {
+  if (p.nelem > 0) {
+    hipLaunchKernelGGL((k_elem_gradients<DIM, NEN, NGP>),
+                       dim3(grid_for(p.nelem * NGP, 256, 65536)), dim3(256), 0, s, p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (p.nnodes == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_assemble_pairs<DIM, NEN, NGP, MAXS>),
+                     dim3(grid_for(p.nnodes * MAXS, 256, 1 << 20)), dim3(256), 0, s, p);
+  return hipGetLastError();
+}
+}  // namespace
+
+int pairs_max_slots(int type) {
+  switch (type) {
+    case GF_Q4: return 9;
+    case GF_Q8: return 21;
+    case GF_BRICK8: return 27;
+    case GF_BRICK20: return 81;
+    default: return 0;
+  }
+}
+
+int64_t pairs_egrad_doubles(int type, int ngauss, int64_t nelem) {
+  ElemInfo ei;
+  if (!elem_info(type, &ei)) return 0;
+  int ngp = 1;
+  for (int d = 0; d < ei.dim; ++d) ngp *= ngauss;
+  return nelem * ngp * (int64_t(ei.nen) * ei.dim + 1);
+}
+
+hipError_t launch_assembly_pairs(hipStream_t s, const AssemblyArgs& a, double* egrad) {
+  ElemInfo ei;
+  if (!elem_info(a.type, &ei) || (a.ngauss != 2 && a.ngauss != 3)) return hipErrorInvalidValue;
+  PArgs p;
+  KArgs& k = p.k;
+  k.conn = a.conn;
+  k.coords = a.coords;
+  k.elemE = a.elemE;
+  k.E = a.E;
+  k.c0 = a.c[0];
+  k.c1 = a.c[1];
+  k.c2 = a.c[2];
+  k.elems = nullptr;
+  k.nce = 0;
+  k.slot = nullptr;
+  k.val = a.val;
+  k.n = a.n;
+  k.bad = a.bad_jac;
+  k.tab = h_off[tab_block(a.type, a.ngauss)];
+  p.egrad = egrad;
+  p.nelem = a.nelem;
+  p.n2e_ptr = a.n2e_ptr;
+  p.n2e = a.n2e;
+  p.rownnz = a.rownnz;
+  p.col = a.col;
+  p.nnodes = a.nnodes;
+  p.bval = a.bval;
+  p.bsptr = a.bsptr;
+  p.binv = a.binv;
+  const bool g3 = a.ngauss == 3;
+  switch (a.type) {
+    case GF_Q4: return g3 ? launch_pairs<2, 4, 9, 9>(s, p) : launch_pairs<2, 4, 4, 9>(s, p);
+    case GF_Q8: return g3 ? launch_pairs<2, 8, 9, 21>(s, p) : launch_pairs<2, 8, 4, 21>(s, p);
+    case GF_BRICK8:
+      return g3 ? launch_pairs<3, 8, 27, 27>(s, p) : launch_pairs<3, 8, 8, 27>(s, p);
+    case GF_BRICK20:
+      return g3 ? launch_pairs<3, 20, 27, 81>(s, p) : launch_pairs<3, 20, 8, 81>(s, p);
+    default: return hipErrorInvalidValue;
+  }
+}
 
 int gather_max_slots(int type) {
   switch (type) {

@@ -43,8 +43,18 @@ struct AssemblyArgs {
   double* bval;
   const int64_t* bsptr;
   const int32_t* binv;
+  // two-phase assembly: ELL columns and the element count
+  const int32_t* col;
+  int64_t nelem;
 };
 hipError_t launch_assembly_color(hipStream_t s, const AssemblyArgs& a);
+// Two-phase assembly (variant 4): the element gradients at the Gauss points
+// into egrad (pairs_egrad_doubles doubles), then one thread per (node,
+// neighbour slot) summing that block over the shared elements; applicable
+// when every node has at most pairs_max_slots(type) neighbours.
+int pairs_max_slots(int type);
+int64_t pairs_egrad_doubles(int type, int ngauss, int64_t nelem);
+hipError_t launch_assembly_pairs(hipStream_t s, const AssemblyArgs& a, double* egrad);
 // Node-gather assembly (variant 3): one launch for the whole mesh, no
 // coloring.  Each thread owns rows of one node, computes the rows of Ke of
 // every element around the node and adds them in LDS, then writes its rows

@@ -183,14 +183,22 @@ int gf_assemble(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, gf_timing* t);
  * cores; 3 (Q4, Q8, Brick8 with at most 9 / 21 / 27 neighbours per node) =
  * node gather, one thread per node row(s) forming the node's rows of every
  * surrounding element's Ke and writing its ELL rows once (no coloring, no
- * read-modify-write).  0 and 1 add the same terms in the same order
- * (bitwise-identical values); 2 and 3 sum in other orders (equal to
- * rounding).  force_variant 0..3 fixes it (GF_ERR_UNSUPPORTED where 3 does
- * not fit); -1 assembles twice with each of 0, 1 and 3 (where it fits) and
- * keeps the fastest (gf_assemble does this on its first call for meshes of
- * 65536 elements or more).  ms_per_variant (may be NULL): GF_ASM_VARIANTS
- * entries, the better of two timings of each variant timed, -1 otherwise. */
-#define GF_ASM_VARIANTS 4
+ * read-modify-write); 4 (forced only; at most 9 / 21 / 27 / 81 neighbours per
+ * node) = two phases, the gradients of every element at its Gauss points
+ * computed once into a buffer the matrix keeps while 4 is its variant
+ * (nelem x ngp x (nen dim + 1) doubles), then one thread per (node,
+ * neighbour node) summing that dim x dim block over the shared elements and
+ * writing it once.  0 and 1 add the same terms in the same order
+ * (bitwise-identical values); 2, 3 and 4 sum in other orders (equal to
+ * rounding).  force_variant 0..4 fixes it (GF_ERR_UNSUPPORTED where 3 or 4
+ * does not fit); -1 assembles twice with each of 0, 1 and 3 (where it fits)
+ * and keeps the fastest (gf_assemble does this on its first call for meshes
+ * of 65536 elements or more).  When
+ * the solver's block-sliced copy of the matrix exists, 3 and 4 write it
+ * beside the ELL values and the others are followed by the copy, inside the
+ * timing.  ms_per_variant (may be NULL): GF_ASM_VARIANTS entries, the better
+ * of two timings of each variant timed, -1 otherwise. */
+#define GF_ASM_VARIANTS 5
 int gf_assemble_tune(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int32_t force_variant,
                      double* ms_per_variant, int32_t* chosen);
 

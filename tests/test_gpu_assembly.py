@@ -94,9 +94,9 @@ def test_pair_major_assembly_is_bitwise_element_major(gpu_ctx, t, nx, ny, nz, ng
     chosen, ms = gpu_ctx.assemble_tune(mesh, K)
     gather = t != gf.GF_BRICK20
     assert chosen in ((0, 1, 3) if gather else (0, 1))
-    assert ms[0] > 0 and ms[1] > 0 and ms[2] == -1.0 and (ms[3] > 0) == gather
+    assert ms[0] > 0 and ms[1] > 0 and ms[2] == -1.0 and (ms[3] > 0) == gather and ms[4] == -1.0
     with pytest.raises(gf.GpufemError):
-        gpu_ctx.assemble_tune(mesh, K, 4)
+        gpu_ctx.assemble_tune(mesh, K, 5)
 
 
 @pytest.mark.parametrize("t,nx,ny,nz,ng", [(gf.GF_BRICK8, 7, 5, 4, 2), (gf.GF_BRICK8, 4, 3, 3, 3),
@@ -132,6 +132,36 @@ def test_gather_assembly_matches_oracle(gpu_ctx, t, nx, ny, nz, ng):
     gpu_ctx.assemble(mesh, K)
     v2, _, _ = compare(gpu_ctx, K, Ko)
     assert np.array_equal(v1, v2)
+
+
+@pytest.mark.parametrize("t,nx,ny,nz,ng", GATHER_CASES + [(gf.GF_BRICK20, 4, 3, 3, 3),
+                                                          (gf.GF_BRICK20, 3, 3, 2, 2)])
+def test_two_phase_assembly_matches_oracle(gpu_ctx, t, nx, ny, nz, ng):
+    """Two phases (element gradients at the Gauss points, then one thread per
+    node block): within 1e-12 of the oracle on distorted elements with
+    per-element moduli, deterministic, and a re-assembly overwrites earlier
+    values (penalty included); with the block-sliced copy built it is written
+    too (the products stay bitwise those of the ELL values)."""
+    g = gf.Grid(t, nx, ny, nz)
+    rng = np.random.default_rng(37)
+    E = rng.uniform(1e5, 3e5, g.nelem)
+    g_, conn, coords, mesh, K, Ko = build_both(gpu_ctx, t, nx, ny, nz, ng, distort=0.15,
+                                               elem_E=E, seed=8)
+    gpu_ctx.assemble_tune(mesh, K, 4)
+    gpu_ctx.assemble(mesh, K)
+    v1, _, _ = compare(gpu_ctx, K, Ko)
+    gpu_ctx.apply_penalty(K, np.arange(0, K.n, 7, dtype=np.int32))
+    gpu_ctx.assemble(mesh, K)
+    v2, _, _ = compare(gpu_ctx, K, Ko)
+    assert np.array_equal(v1, v2)
+    bsell = 7 if g.dim == 2 else 8
+    gpu_ctx.tune(K, bsell)
+    gpu_ctx.assemble(mesh, K)
+    x = rng.normal(size=K.n)
+    yb, _ = gpu_ctx.spmv(K, x)
+    gpu_ctx.tune(K, 0)
+    y0, _ = gpu_ctx.spmv(K, x)
+    assert np.array_equal(yb, y0)
 
 
 def test_gather_assembly_rejects_brick20(gpu_ctx):
@@ -179,3 +209,8 @@ def test_degenerate_element_reported(gpu_ctx):
     with pytest.raises(gf.GpufemError) as e:
         gpu_ctx.assemble(mesh, K)
     assert e.value.code == gf.GF_ERR_DEGENERATE
+    for v in (3, 4):
+        gpu_ctx.assemble_tune(mesh, K, v)
+        with pytest.raises(gf.GpufemError) as e:
+            gpu_ctx.assemble(mesh, K)
+        assert e.value.code == gf.GF_ERR_DEGENERATE

@@ -1,10 +1,15 @@
 """The trained warm-start network (gpufem_amd/weights/) on held-out members
-of the Brick8 cantilever family: every warm-started solve meets the same true
-residual as zero-start PCG; what the network gains is a better initial
-guess than the method without it (its untrained output, the beam-theory
-estimate, and zero start); the iterations it saves are bounded by the
-residual test (a few percent at this accuracy), so the test bounds the
-iteration ratio instead of asking for a large saving; and the GPU inference
+of the Brick8 cantilever family.  Measured: the network's prediction is a
+better initial guess than the method without it (its untrained output, the
+beam-theory estimate, and zero start) by the L2 error, but its residual
+||b - K x0|| is larger than ||b|| (the small-scale roughness of a
+regression network costs more residual than its large-scale accuracy
+saves), so with a stopping test relative to ||b|| it cannot save Jacobi-PCG
+iterations.  The solver therefore takes the guess guarded (use_x0 = 2): it
+is kept only when its residual is below ||b||, else the solve is the
+zero-start one, bitwise.  The test asserts what the network buys (the
+closer guess), that a warm-started solve never takes more iterations than
+zero start and meets the same true residual, and that the GPU inference
 equals the CPU network oracle on the trained weights."""
 import os
 
@@ -28,20 +33,25 @@ def trained(gpu_ctx):
     return net, c_u, meta, L.Family()
 
 
-def test_warm_start_meets_residual_and_saves_iterations(gpu_ctx, trained):
+def test_warm_start_meets_residual_and_never_costs_iterations(gpu_ctx, trained):
     net, c_u, meta, fam = trained
     ks = [L.HOLDOUT_BASE + k for k in range(6)]
     recs = L.evaluate(gpu_ctx, fam, net, c_u, ks)
     for r in recs:
         assert r["zero_true_relres"] <= 1e-8 and r["warm_true_relres"] <= 1e-8, r
         assert r["x0_rel_err"] < 0.25, r
-        assert r["solution_rel_diff"] <= 1e-8, r
+        # a guarded guess never makes the solve longer; a rejected one gives
+        # the zero-start solve exactly
+        assert r["warm_it"] <= r["zero_it"], r
+        if r["warm_x0_rejected"]:
+            assert r["warm_it"] == r["zero_it"] and r["solution_rel_diff"] == 0.0, r
+        else:
+            assert r["solution_rel_diff"] <= 1e-8, r
     s = L.summarize(recs)
-    # what the trained network gains on members it never saw: a closer
+    # what the trained network buys on members it never saw: a closer
     # initial guess than its own untrained output (the beam estimate)
     assert s["x0_rel_err_mean"] < 0.8 * s["base_rel_err_mean"] < 0.8, s
-    # and the warm start costs no more than a few percent of iterations
-    assert s["iteration_ratio"] <= 1.05, s
+    assert s["iteration_ratio"] <= 1.0, s
 
 
 def test_inference_matches_network_oracle(gpu_ctx, trained):

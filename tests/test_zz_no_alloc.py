@@ -19,8 +19,6 @@ MAX_ROWS = 1 << 20
 
 def noalloc(ctx, K, d_b, d_x, **kw):
     fn = ctx.lib.gf_pcg_noalloc
-    fn.restype = C.c_int
-    fn.argtypes = [C.c_void_p] * 5 + [C.POINTER(_lib.SolveStats)]
     o = gf.solve_opts(device_ptrs=1, **kw)
     st = _lib.SolveStats()
     code = fn(ctx.handle, K.handle, d_b.ptr, d_x.ptr, C.byref(o), C.byref(st))
