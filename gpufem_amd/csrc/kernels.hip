@@ -445,24 +445,49 @@ struct GArgs {
   const int32_t* binv;
 };
 
-template <int DIM, int NEN, int NGP, int MAXS, int RPT>
+// A unit is RPT rows of one node (UPN units per node).  LPU lanes share a
+// unit: lane j of the unit takes the node's incidences j, j + LPU, ... and the
+// LPU lanes add their element's rows into the unit's LDS accumulators in LPU
+// masked phases, so every accumulator receives the elements in ascending
+// incidence order (the sums do not depend on LPU).  UPW units per 64-lane
+// workgroup: LPU > 1 divides the LDS per workgroup by LPU and lets more
+// wavefronts share a CU.
+// With RPT == DIM (every row of the node in one unit) the lane forms the
+// products T_pq = sum_g (w detJ E dNa/dx_p)(dNb/dx_q) per neighbour b and the
+// elastic combinations once per element; the element's nodes b are taken BCH
+// at a time (the Gauss-point geometry is recomputed per chunk), which bounds
+// the accumulator registers.  With RPT < DIM (one row per unit) the row's
+// entries are accumulated directly and BCH = NEN.
+template <int DIM, int NEN, int NGP, int MAXS, int RPT, int LPU, int UPW, int BCH>
 __global__ __launch_bounds__(64) void k_assemble_gather(GArgs ga) {
-  constexpr int UPN = DIM / RPT;  // threads per node
+  static_assert(LPU * UPW <= 64, "a workgroup is one wavefront");
+  static_assert(NEN % BCH == 0 && (RPT == DIM || BCH == NEN), "neighbour chunks");
+  constexpr int UPN = DIM / RPT;  // units per node
   constexpr int ACC = MAXS * RPT * DIM;
-  __shared__ double s_acc[ACC * 64];
+  __shared__ double s_acc[ACC * UPW];
   const KArgs& a = ga.k;
-  const int lane = threadIdx.x;
+  const int lu = threadIdx.x / LPU;  // unit of this lane inside the workgroup
+  const int lj = threadIdx.x % LPU;  // lane of the unit
   const int64_t units = ga.nnodes * UPN;
-  // grid-stride over units; every lane runs the same trip count (LDS is per lane)
-  for (int64_t u0 = int64_t(blockIdx.x) * 64; u0 < units; u0 += int64_t(gridDim.x) * 64) {
-    const int64_t u = u0 + lane;
-    const bool live = u < units;
+  // grid-stride over units; every lane runs the same trip count (LDS is per unit)
+  for (int64_t u0 = int64_t(blockIdx.x) * UPW; u0 < units; u0 += int64_t(gridDim.x) * UPW) {
+    const int64_t u = u0 + lu;
+    const bool live = lu < UPW && u < units;
     const int64_t v = live ? u / UPN : 0;
     const int r0 = live ? int(u % UPN) * RPT : 0;
-#pragma unroll 4
-    for (int q = 0; q < ACC; ++q) s_acc[q * 64 + lane] = 0.0;
+    if (lu < UPW)
+      for (int q = lj; q < ACC; q += LPU) s_acc[q * UPW + lu] = 0.0;
     const int64_t k0 = live ? ga.n2e_ptr[v] : 0, k1 = live ? ga.n2e_ptr[v + 1] : 0;
-    for (int64_t kk = k0; kk < k1; ++kk) {
+    // trip count: the largest over the wavefront, so the phases below are
+    // reached by every lane together
+    int trips = int((k1 - k0 + LPU - 1) / LPU);
+    if constexpr (LPU > 1) {
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) trips = max(trips, __shfl_xor(trips, off));
+    }
+    for (int t = 0; t < trips; ++t) {
+      const int64_t kk = k0 + int64_t(t) * LPU + lj;
+      if (kk >= k1) continue;  // this lane's share of the node is done
       const int32_t packed = ga.n2e[kk];
       const int64_t e = packed >> 5;
       const int la = packed & 31;
@@ -474,107 +499,137 @@ __global__ __launch_bounds__(64) void k_assemble_gather(GArgs ga) {
         for (int d = 0; d < DIM; ++d) x[n][d] = a.coords[node * DIM + d];
       }
       const double Ee = a.elemE ? a.elemE[e] : a.E;
-      double kl[NEN][RPT][DIM];
+      const uint8_t* sl = a.slot + (e * NEN + la) * NEN;
+      for (int c0 = 0; c0 < NEN; c0 += BCH) {
+        double kl[BCH][RPT][DIM];
 #pragma unroll
-      for (int b = 0; b < NEN; ++b)
+        for (int b = 0; b < BCH; ++b)
 #pragma unroll
-        for (int r = 0; r < RPT; ++r)
+          for (int r = 0; r < RPT; ++r)
 #pragma unroll
-          for (int j = 0; j < DIM; ++j) kl[b][r][j] = 0.0;
-      for (int g = 0; g < NGP; ++g) {
-        const double* dN = c_tab + a.tab + g * DIM * NEN;
-        double J[DIM][DIM];
+            for (int j = 0; j < DIM; ++j) kl[b][r][j] = 0.0;
+        for (int g = 0; g < NGP; ++g) {
+          const double* dN = c_tab + a.tab + g * DIM * NEN;
+          double J[DIM][DIM];
 #pragma unroll
-        for (int i = 0; i < DIM; ++i)
+          for (int i = 0; i < DIM; ++i)
 #pragma unroll
-          for (int j = 0; j < DIM; ++j) {
-            double acc = 0.0;
-#pragma unroll
-            for (int n = 0; n < NEN; ++n) acc += dN[i * NEN + n] * x[n][j];
-            J[i][j] = acc;
-          }
-        double Ji[DIM][DIM];
-        const double det = det_inv<DIM>(J, Ji);
-        if (!(det > 0.0)) *a.bad = 1;
-        const double sc = c_tab[a.tab + NGP * DIM * NEN + g] * det * Ee;
-        double gA[DIM];
-#pragma unroll
-        for (int i = 0; i < DIM; ++i) {
-          double acc = 0.0;
-#pragma unroll
-          for (int k = 0; k < DIM; ++k) acc += Ji[i][k] * dN[k * NEN + la];
-          gA[i] = acc;
-        }
-        if constexpr (DIM == 2 && RPT == 2) {
-          // both rows of the node: accumulate the four products
-          // T_pq = sum_g (sc gA_p)(gB_q) per node b; the plane-stress
-          // combinations are formed once per element below
-          const double sA[2] = {sc * gA[0], sc * gA[1]};
-#pragma unroll
-          for (int b = 0; b < NEN; ++b) {
-            double gB[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-              gB[i] = Ji[i][0] * dN[b] + Ji[i][1] * dN[NEN + b];
-#pragma unroll
-            for (int p = 0; p < 2; ++p)
-#pragma unroll
-              for (int q = 0; q < 2; ++q) kl[b][p][q] += sA[p] * gB[q];
-          }
-        } else {
-#pragma unroll
-          for (int b = 0; b < NEN; ++b) {
-            double gB[DIM];
-#pragma unroll
-            for (int i = 0; i < DIM; ++i) {
+            for (int j = 0; j < DIM; ++j) {
               double acc = 0.0;
 #pragma unroll
-              for (int k = 0; k < DIM; ++k) acc += Ji[i][k] * dN[k * NEN + b];
-              gB[i] = acc;
+              for (int n = 0; n < NEN; ++n) acc += dN[i * NEN + n] * x[n][j];
+              J[i][j] = acc;
             }
-            double dot = 0.0;
+          double Ji[DIM][DIM];
+          const double det = det_inv<DIM>(J, Ji);
+          if (!(det > 0.0)) *a.bad = 1;
+          const double sc = c_tab[a.tab + NGP * DIM * NEN + g] * det * Ee;
+          double gA[DIM];
 #pragma unroll
-            for (int i = 0; i < DIM; ++i) dot += gA[i] * gB[i];
+          for (int i = 0; i < DIM; ++i) {
+            double acc = 0.0;
 #pragma unroll
-            for (int r = 0; r < RPT; ++r) {
-              const int i = r0 + r;
+            for (int k = 0; k < DIM; ++k) acc += Ji[i][k] * dN[k * NEN + la];
+            gA[i] = acc;
+          }
+          if constexpr (RPT == DIM) {
+            // every row of the node: accumulate the products T_pq; the
+            // elastic combinations are formed once per element below
+            double sA[DIM];
 #pragma unroll
-              for (int j = 0; j < DIM; ++j) {
-                double kij;
-                if constexpr (DIM == 2) {
-                  if (i == j) kij = a.c0 * gA[i] * gB[i] + a.c2 * gA[1 - i] * gB[1 - i];
-                  else kij = a.c1 * gA[i] * gB[j] + a.c2 * gA[j] * gB[i];
-                } else {
-                  if (i == j) kij = a.c0 * gA[i] * gB[i] + a.c2 * (dot - gA[i] * gB[i]);
-                  else kij = a.c1 * gA[i] * gB[j] + a.c2 * gA[j] * gB[i];
+            for (int p = 0; p < DIM; ++p) sA[p] = sc * gA[p];
+#pragma unroll
+            for (int b = 0; b < BCH; ++b) {
+              double gB[DIM];
+#pragma unroll
+              for (int i = 0; i < DIM; ++i) {
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < DIM; ++k) acc += Ji[i][k] * dN[k * NEN + c0 + b];
+                gB[i] = acc;
+              }
+#pragma unroll
+              for (int p = 0; p < DIM; ++p)
+#pragma unroll
+                for (int q = 0; q < DIM; ++q) kl[b][p][q] += sA[p] * gB[q];
+            }
+          } else {
+#pragma unroll
+            for (int b = 0; b < BCH; ++b) {
+              double gB[DIM];
+#pragma unroll
+              for (int i = 0; i < DIM; ++i) {
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < DIM; ++k) acc += Ji[i][k] * dN[k * NEN + c0 + b];
+                gB[i] = acc;
+              }
+              double dot = 0.0;
+#pragma unroll
+              for (int i = 0; i < DIM; ++i) dot += gA[i] * gB[i];
+#pragma unroll
+              for (int r = 0; r < RPT; ++r) {
+                const int i = r0 + r;
+#pragma unroll
+                for (int j = 0; j < DIM; ++j) {
+                  double kij;
+                  if constexpr (DIM == 2) {
+                    if (i == j) kij = a.c0 * gA[i] * gB[i] + a.c2 * gA[1 - i] * gB[1 - i];
+                    else kij = a.c1 * gA[i] * gB[j] + a.c2 * gA[j] * gB[i];
+                  } else {
+                    if (i == j) kij = a.c0 * gA[i] * gB[i] + a.c2 * (dot - gA[i] * gB[i]);
+                    else kij = a.c1 * gA[i] * gB[j] + a.c2 * gA[j] * gB[i];
+                  }
+                  kl[b][r][j] += sc * kij;
                 }
-                kl[b][r][j] += sc * kij;
               }
             }
           }
         }
-      }
-      if constexpr (DIM == 2 && RPT == 2) {
-        // K_ii = c0 T_ii + c2 T_kk (k the other direction), K_ij = c1 T_ij + c2 T_ji
+        if constexpr (RPT == DIM) {
+          // K_ii = c0 T_ii + c2 (sum of T_kk, k != i), K_ij = c1 T_ij + c2 T_ji
 #pragma unroll
-        for (int b = 0; b < NEN; ++b) {
-          const double t00 = kl[b][0][0], t01 = kl[b][0][1], t10 = kl[b][1][0], t11 = kl[b][1][1];
-          kl[b][0][0] = a.c0 * t00 + a.c2 * t11;
-          kl[b][1][1] = a.c0 * t11 + a.c2 * t00;
-          kl[b][0][1] = a.c1 * t01 + a.c2 * t10;
-          kl[b][1][0] = a.c1 * t10 + a.c2 * t01;
+          for (int b = 0; b < BCH; ++b) {
+            double T[DIM][DIM];
+#pragma unroll
+            for (int p = 0; p < DIM; ++p)
+#pragma unroll
+              for (int q = 0; q < DIM; ++q) T[p][q] = kl[b][p][q];
+#pragma unroll
+            for (int p = 0; p < DIM; ++p)
+#pragma unroll
+              for (int q = 0; q < DIM; ++q) {
+                if (p == q) {
+                  double o = 0.0;
+#pragma unroll
+                  for (int k = 0; k < DIM; ++k)
+                    if (k != p) o += T[k][k];
+                  kl[b][p][p] = a.c0 * T[p][p] + a.c2 * o;
+                } else {
+                  kl[b][p][q] = a.c1 * T[p][q] + a.c2 * T[q][p];
+                }
+              }
+          }
+        }
+        int sb[BCH];
+#pragma unroll
+        for (int b = 0; b < BCH; ++b) sb[b] = sl[c0 + b];
+#pragma unroll
+        for (int p = 0; p < LPU; ++p) {
+          if constexpr (LPU > 1) __builtin_amdgcn_wave_barrier();
+          if (lj == p) {
+#pragma unroll
+            for (int b = 0; b < BCH; ++b)
+#pragma unroll
+              for (int r = 0; r < RPT; ++r)
+#pragma unroll
+                for (int j = 0; j < DIM; ++j)
+                  s_acc[((sb[b] * RPT + r) * DIM + j) * UPW + lu] += kl[b][r][j];
+          }
         }
       }
-      const uint8_t* sl = a.slot + (e * NEN + la) * NEN;
-#pragma unroll
-      for (int b = 0; b < NEN; ++b) {
-        const int s = sl[b];
-#pragma unroll
-        for (int r = 0; r < RPT; ++r)
-#pragma unroll
-          for (int j = 0; j < DIM; ++j) s_acc[((s * RPT + r) * DIM + j) * 64 + lane] += kl[b][r][j];
-      }
     }
+    if constexpr (LPU > 1) __builtin_amdgcn_wave_barrier();
     if (live) {
       // the block-sliced copy: block s of node row v sits at slot
       // bsptr[slice] + s, lane pos & 63 of the slice, value (row, column)
@@ -590,10 +645,10 @@ __global__ __launch_bounds__(64) void k_assemble_gather(GArgs ga) {
       for (int r = 0; r < RPT; ++r) {
         const int64_t row = v * DIM + r0 + r;
         const int32_t ns = ga.rownnz[row] / DIM;
-        for (int s = 0; s < ns; ++s)
+        for (int s = lj; s < ns; s += LPU)
 #pragma unroll
           for (int j = 0; j < DIM; ++j) {
-            const double kv = s_acc[((s * RPT + r) * DIM + j) * 64 + lane];
+            const double kv = s_acc[((s * RPT + r) * DIM + j) * UPW + lu];
             a.val[row + (int64_t(s) * DIM + j) * a.n] = kv;
             if (ga.bval) ga.bval[bsell_vidx(DIM * DIM, bb0 + s, (r0 + r) * DIM + j, bl)] = kv;
           }
@@ -1028,17 +1083,30 @@ __global__ __launch_bounds__(256) void k_assemble_pairs(PArgs pa) {
 }
 
 namespace {
-template <int DIM, int NEN, int NGP, int MAXS>
-hipError_t launch_gather(hipStream_t s, const GArgs& g) {
-  // 2D: both rows of a node per thread (the geometry is shared; measured
-  // faster than one row per thread despite twice the LDS per workgroup);
-  // 3D: one row per thread (three rows of accumulators exceed the LDS budget)
-  constexpr int RPT = DIM == 2 ? 2 : 1;
+template <int DIM, int NEN, int NGP, int MAXS, int RPT, int LPU, int UPW, int BCH>
+hipError_t launch_gather_cfg(hipStream_t s, const GArgs& g) {
   const int64_t units = g.nnodes * (DIM / RPT);
   if (units == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_assemble_gather<DIM, NEN, NGP, MAXS, RPT>),
-                     dim3(grid_for(units, 64, 65536)), dim3(64), 0, s, g);
+  hipLaunchKernelGGL((k_assemble_gather<DIM, NEN, NGP, MAXS, RPT, LPU, UPW, BCH>),
+                     dim3(grid_for(units, UPW, 65536)), dim3(64), 0, s, g);
   return hipGetLastError();
+}
+
+// Unit shapes (measured on MI355X, gather kernel alone, LDS and registers
+// from the compiler's resource report):
+//  2D: node units (both rows), 2 lanes per node; Q8 1024x512 0.62 ms (one lane
+//      per node: 0.86 ms, 43 KB of LDS per workgroup, 1 wavefront per SIMD).
+//  Brick8: node units, 4 lanes per node, 16 nodes per workgroup, neighbours in
+//      chunks of 4 (223 VGPRs): 128^3 4.6 ms; row units (one row per unit,
+//      each element's geometry once per row): 6.3-6.6 ms with 2 lanes per row,
+//      9.7 ms with one lane per row.
+//  Brick20: row units, 4 lanes per row (256 VGPRs + AGPRs): 48^3 8.1-8.7 ms;
+//      node units need chunks of at most 4 neighbours for their registers and
+//      recompute the 20-node geometry per chunk (10.8-20.5 ms).
+template <int NEN, int NGP, int MAXS>
+hipError_t launch_gather2(hipStream_t s, const GArgs& g) {
+  if constexpr (NEN == 4) return launch_gather_cfg<2, NEN, NGP, MAXS, 2, 2, 32, NEN>(s, g);
+  else return launch_gather_cfg<2, NEN, NGP, MAXS, 2, 2, 30, NEN>(s, g);
 }
 }  // namespace
 
@@ -1122,7 +1190,8 @@ int gather_max_slots(int type) {
     case GF_Q4: return 9;
     case GF_Q8: return 21;
     case GF_BRICK8: return 27;
-    default: return 0;  // Brick20: 81 slots x 3 doubles per row exceed the LDS budget
+    case GF_BRICK20: return 81;
+    default: return 0;
   }
 }
 
@@ -1155,9 +1224,14 @@ hipError_t launch_assembly_gather(hipStream_t s, const AssemblyArgs& a) {
   g.binv = a.binv;
   const bool g3 = a.ngauss == 3;
   switch (a.type) {
-    case GF_Q4: return g3 ? launch_gather<2, 4, 9, 9>(s, g) : launch_gather<2, 4, 4, 9>(s, g);
-    case GF_Q8: return g3 ? launch_gather<2, 8, 9, 21>(s, g) : launch_gather<2, 8, 4, 21>(s, g);
-    case GF_BRICK8: return g3 ? launch_gather<3, 8, 27, 27>(s, g) : launch_gather<3, 8, 8, 27>(s, g);
+    case GF_Q4: return g3 ? launch_gather2<4, 9, 9>(s, g) : launch_gather2<4, 4, 9>(s, g);
+    case GF_Q8: return g3 ? launch_gather2<8, 9, 21>(s, g) : launch_gather2<8, 4, 21>(s, g);
+    case GF_BRICK8:
+      return g3 ? launch_gather_cfg<3, 8, 27, 27, 3, 4, 16, 4>(s, g)
+                : launch_gather_cfg<3, 8, 8, 27, 3, 4, 16, 4>(s, g);
+    case GF_BRICK20:
+      return g3 ? launch_gather_cfg<3, 20, 27, 81, 1, 4, 16, 20>(s, g)
+                : launch_gather_cfg<3, 20, 8, 81, 1, 4, 16, 20>(s, g);
     default: return hipErrorInvalidValue;
   }
 }

@@ -92,9 +92,8 @@ def test_pair_major_assembly_is_bitwise_element_major(gpu_ctx, t, nx, ny, nz, ng
     v1, c1, r1 = gpu_ctx.download(K)
     assert np.array_equal(v0, v1) and np.array_equal(c0, c1)
     chosen, ms = gpu_ctx.assemble_tune(mesh, K)
-    gather = t != gf.GF_BRICK20
-    assert chosen in ((0, 1, 3) if gather else (0, 1))
-    assert ms[0] > 0 and ms[1] > 0 and ms[2] == -1.0 and (ms[3] > 0) == gather and ms[4] == -1.0
+    assert chosen in (0, 1, 3)
+    assert ms[0] > 0 and ms[1] > 0 and ms[2] == -1.0 and ms[3] > 0 and ms[4] == -1.0
     with pytest.raises(gf.GpufemError):
         gpu_ctx.assemble_tune(mesh, K, 5)
 
@@ -114,12 +113,14 @@ GATHER_CASES = [(gf.GF_Q4, 64, 32, 0, 2), (gf.GF_Q4, 7, 5, 0, 3), (gf.GF_Q8, 24,
                 (gf.GF_Q8, 9, 4, 0, 2), (gf.GF_BRICK8, 8, 6, 5, 2), (gf.GF_BRICK8, 4, 4, 3, 3)]
 
 
-@pytest.mark.parametrize("t,nx,ny,nz,ng", GATHER_CASES)
+@pytest.mark.parametrize("t,nx,ny,nz,ng", GATHER_CASES + [(gf.GF_BRICK20, 4, 3, 3, 3),
+                                                          (gf.GF_BRICK20, 5, 3, 2, 2)])
 def test_gather_assembly_matches_oracle(gpu_ctx, t, nx, ny, nz, ng):
-    """Node gather (one thread per node row(s), element contributions summed
-    in element order, every live slot written once): within 1e-12 of the
-    oracle on distorted elements with per-element moduli, deterministic, and
-    a re-assembly overwrites earlier values (penalty included)."""
+    """Node gather (lanes of a node or node row share its elements, element
+    contributions summed in element order, every live slot written once):
+    within 1e-12 of the oracle on distorted elements with per-element moduli,
+    deterministic, and a re-assembly overwrites earlier values (penalty
+    included)."""
     g = gf.Grid(t, nx, ny, nz)
     rng = np.random.default_rng(31)
     E = rng.uniform(1e5, 3e5, g.nelem)
@@ -162,12 +163,6 @@ def test_two_phase_assembly_matches_oracle(gpu_ctx, t, nx, ny, nz, ng):
     gpu_ctx.tune(K, 0)
     y0, _ = gpu_ctx.spmv(K, x)
     assert np.array_equal(yb, y0)
-
-
-def test_gather_assembly_rejects_brick20(gpu_ctx):
-    g, conn, coords, mesh, K, Ko = build_both(gpu_ctx, gf.GF_BRICK20, 2, 2, 2, 2)
-    with pytest.raises(gf.GpufemError):
-        gpu_ctx.assemble_tune(mesh, K, 3)
 
 
 def test_matrix_core_assembly_rejects_2d(gpu_ctx):

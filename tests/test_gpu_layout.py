@@ -14,7 +14,8 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("t,dims,bsell", [(gf.GF_Q8, (20, 10), 9), (gf.GF_Q4, (33, 17), 7),
                                           (gf.GF_BRICK8, (7, 6, 5), 10),
-                                          (gf.GF_BRICK8, (5, 4, 3), 8)])
+                                          (gf.GF_BRICK8, (5, 4, 3), 8),
+                                          (gf.GF_BRICK20, (4, 3, 3), 10)])
 def test_gather_writes_block_sliced_copy(gpu_ctx, t, dims, bsell):
     """With the copy built, the node gather writes it beside the ELL values
     and the penalty updates its diagonal: its products stay bitwise equal to
