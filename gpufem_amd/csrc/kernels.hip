@@ -631,9 +631,9 @@ __global__ __launch_bounds__(64) void k_assemble_gather(GArgs ga) {
     }
     if constexpr (LPU > 1) __builtin_amdgcn_wave_barrier();
     if (live) {
-      // the block-sliced copy: block s of node row v sits at slot
-      // bsptr[slice] + s, lane pos & 63 of the slice, value (row, column)
-      // of the block at index row * DIM + column
+      // the block-sliced copy: block s of node v sits at slot bsptr[slice] + s,
+      // lane pos & 63 of the slice, value (row, column) of the block at index
+      // row * DIM + column
       int64_t bb0 = 0;
       int bl = 0;
       if (ga.bval) {
@@ -641,17 +641,43 @@ __global__ __launch_bounds__(64) void k_assemble_gather(GArgs ga) {
         bb0 = ga.bsptr[pos >> 6];
         bl = pos & 63;
       }
+      if constexpr (RPT == DIM) {
+        // every row of the node: its blocks go out as 16-byte pairs
+        constexpr int DD = DIM * DIM;
+        const int32_t ns = ga.rownnz[v * DIM] / DIM;
+        for (int s = lj; s < ns; s += LPU) {
+          const double* sa = s_acc + int64_t(s) * DD * UPW + lu;
 #pragma unroll
-      for (int r = 0; r < RPT; ++r) {
-        const int64_t row = v * DIM + r0 + r;
-        const int32_t ns = ga.rownnz[row] / DIM;
-        for (int s = lj; s < ns; s += LPU)
+          for (int j = 0; j < DIM; ++j)
 #pragma unroll
-          for (int j = 0; j < DIM; ++j) {
-            const double kv = s_acc[((s * RPT + r) * DIM + j) * UPW + lu];
-            a.val[row + (int64_t(s) * DIM + j) * a.n] = kv;
-            if (ga.bval) ga.bval[bsell_vidx(DIM * DIM, bb0 + s, (r0 + r) * DIM + j, bl)] = kv;
+            for (int r = 0; r < DIM; ++r)
+              a.val[v * DIM + r + (int64_t(s) * DIM + j) * a.n] = sa[(r * DIM + j) * UPW];
+          if (ga.bval) {
+            typedef double P2 __attribute__((ext_vector_type(2)));
+            double* blk = ga.bval + (bb0 + s) * DD * 64;
+#pragma unroll
+            for (int kp = 0; kp < DD / 2; ++kp) {
+              P2 pr;
+              pr.x = sa[(2 * kp) * UPW];
+              pr.y = sa[(2 * kp + 1) * UPW];
+              reinterpret_cast<P2*>(blk)[kp * 64 + bl] = pr;
+            }
+            if constexpr (DD & 1) blk[(DD - 1) * 64 + bl] = sa[(DD - 1) * UPW];
           }
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+          const int64_t row = v * DIM + r0 + r;
+          const int32_t ns = ga.rownnz[row] / DIM;
+          for (int s = lj; s < ns; s += LPU)
+#pragma unroll
+            for (int j = 0; j < DIM; ++j) {
+              const double kv = s_acc[((s * RPT + r) * DIM + j) * UPW + lu];
+              a.val[row + (int64_t(s) * DIM + j) * a.n] = kv;
+              if (ga.bval) ga.bval[bsell_vidx(DIM * DIM, bb0 + s, (r0 + r) * DIM + j, bl)] = kv;
+            }
+        }
       }
     }
   }
@@ -1087,8 +1113,10 @@ template <int DIM, int NEN, int NGP, int MAXS, int RPT, int LPU, int UPW, int BC
 hipError_t launch_gather_cfg(hipStream_t s, const GArgs& g) {
   const int64_t units = g.nnodes * (DIM / RPT);
   if (units == 0) return hipSuccess;
+  // one tile of UPW units per workgroup: a wavefront ends after its stores
+  // instead of waiting for them before the loads of a next tile
   hipLaunchKernelGGL((k_assemble_gather<DIM, NEN, NGP, MAXS, RPT, LPU, UPW, BCH>),
-                     dim3(grid_for(units, UPW, 65536)), dim3(64), 0, s, g);
+                     dim3(grid_for(units, UPW, 1 << 30)), dim3(64), 0, s, g);
   return hipGetLastError();
 }
 
