@@ -52,8 +52,10 @@ def ke_flops_per_element(dim, nen, ngp):
     return ngp * (2 * s * s * ne + 2 * s * ne * ne)
 
 
-def cpu_baseline(p, gpu_iters, budget_s):
-    """FP64 restatement of the reference OpenMP path on the host cores."""
+def cpu_baseline(p, gpu_iters, budget_s, full=False):
+    """FP64 restatement of the reference OpenMP path on the host cores.  The
+    PCG is a bounded sample extrapolated to the GPU's iteration count, or
+    (full) the whole solve to 1e-8, reporting its progress as it goes."""
     import oracle as O
     threads = O.default_threads()
     t0 = time.perf_counter()
@@ -64,6 +66,23 @@ def cpu_baseline(p, gpu_iters, budget_s):
     _, t_num = O.assemble(p.dim, p.ngauss, p.conn, p.coords, p.E, p.nu, elem_E=p.elem_E,
                           skip_zero=False, nthreads=threads, into=K)
     O.apply_penalty(K, p.dim, p.supports)
+    if full:
+        _, it, rel, t_pcg = O.pcg_jacobi(K, p.f, rtol=1e-8, maxit=1000000, nthreads=threads,
+                                         progress_every=100)
+        return {
+            "value": p.ndof / t_num,
+            "unit": "DOF/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": ("full %s: colored OpenMP assembly (numeric pass timed, %.3f s; with "
+                       "insertion %.3f s) + Jacobi-PCG from zero to 1e-8 measured whole: %d "
+                       "iterations, %.1f s" % (p.name, t_num, t_full, it, t_pcg)),
+            "assembly_s": t_num,
+            "pcg_s_per_iter": t_pcg / max(it, 1),
+            "pcg_to_1e8_s": t_pcg,
+            "pcg_iterations": it,
+            "pcg_true_relres": rel,
+        }
     # bounded PCG sample: iterations until about half the remaining budget is spent
     spent = time.perf_counter() - t0
     probe_it = 5
@@ -135,6 +154,8 @@ def main():
     ap.add_argument("--workload", default="brick8_cube_128")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=25.0)
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="time the host PCG to 1e-8 whole instead of a bounded sample")
     ap.add_argument("--schedule", default="graph",
                     choices=["kernels", "graph", "persistent"])
     ap.add_argument("--compare", default="all", choices=["all", "none"],
@@ -260,6 +281,12 @@ def main():
                             "fallbacks": st.fallbacks}
             log("%s: %d it, %.3f s, true relres %.2e" % (name, st.iterations,
                                                          st.ms_device / 1e3, st.true_relres))
+    if args.compare == "all" and "error" not in solves.get("fp64_persistent", {"error": 1}):
+        # what the persistent schedule's three grid barriers per iteration cost
+        us, nbp = ctx.profile_barrier(K)
+        solves["persistent_barrier"] = {"us_per_barrier": us, "workgroups": nbp,
+                                        "us_per_iteration": 3 * us}
+        log("persistent schedule: %.2f us per grid barrier (%d workgroups)" % (us, nbp))
     if args.compare == "all":
         # the elimination alternative to the penalty, same mesh and load
         ctx.assemble(mesh, K)
@@ -382,7 +409,8 @@ def main():
     del K, mesh
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
-        out["cpu_baseline"] = cpu_baseline(p, int(round(it_mean)), args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(p, int(round(it_mean)), args.cpu_budget,
+                                           full=args.cpu_full)
     print(json.dumps(out), flush=True)
 
 

@@ -286,6 +286,15 @@ class Context:
         self._check(self.lib.gf_profile_spmv(self.handle, ell.handle, nrep, C.byref(ms)))
         return ms.value
 
+    def profile_barrier(self, ell, nbar=3000):
+        """Microseconds per grid barrier of the persistent schedule, and the
+        persistent grid's workgroup count."""
+        us = C.c_double()
+        nb = C.c_int32()
+        self._check(self.lib.gf_profile_barrier(self.handle, ell.handle, nbar, C.byref(us),
+                                                C.byref(nb)))
+        return us.value, nb.value
+
     def profile_pcg(self, ell, b, iters=200):
         """Average SpMV kernel time and iteration time (ms) over iters PCG
         iterations from zero for b (events around each SpMV)."""

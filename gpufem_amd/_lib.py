@@ -126,6 +126,7 @@ SIGNATURES = {
                                C.c_float, _P, _P, _P, _I32]),
     "gf_nn_conv3d_wgrad": (C.c_int, [_P, _P, _I32, _P, _I32, _P, _P, _I32, _I32, _P, _P]),
     "gf_pcg_noalloc": (C.c_int, [_P, _P, _P, _P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "gf_profile_barrier": (C.c_int, [_P, _P, _I32, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
     "gf_dev_alloc": (C.c_int, [_P, _I64, C.POINTER(_P)]),
     "gf_dev_free": (C.c_int, [_P, _P]),
     "gf_copy_h2d": (C.c_int, [_P, _P, _P, _I64]),

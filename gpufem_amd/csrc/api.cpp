@@ -100,6 +100,7 @@ int gf_create(gf_ctx** out, int32_t device) {
   hipDeviceProp_t prop;
   GF_HIP_CHECK(c.get(), hipGetDeviceProperties(&prop, device));
   c->num_cu = prop.multiProcessorCount;
+  c->lds_per_cu = int(prop.maxSharedMemoryPerMultiProcessor);
   c->name = prop.gcnArchName;
   GF_HIP_CHECK(c.get(), hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   GF_HIP_CHECK(c.get(), hipEventCreate(&c->ev_start));

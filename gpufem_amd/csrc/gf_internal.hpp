@@ -44,6 +44,7 @@ struct gf_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
   int num_cu = 0;
+  int lds_per_cu = 0;  // bytes of LDS per compute unit
   std::string name;
   std::string err;
   // reduction scratch: per-block partial sums (doubles)

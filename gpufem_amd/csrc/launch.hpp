@@ -211,6 +211,10 @@ hipError_t launch_pcg_iteration_timed(hipStream_t s, const PcgBuffers& B, int pa
                                       hipEvent_t spmv_start, hipEvent_t spmv_stop);
 // co-resident workgroups for the persistent kernel of this variant (0: unavailable)
 int persistent_blocks(int variant, int num_cu);
+// nbar grid barriers of the persistent schedule on a cooperative grid of
+// nblocks workgroups; barrier_aborted reads whether a wait hit its spin limit
+hipError_t launch_barrier_probe(hipStream_t s, void* barrier, int nblocks, int nbar);
+int barrier_aborted(hipStream_t s, const void* barrier, int* aborted);
 // all remaining iterations in one cooperative launch
 hipError_t launch_pcg_persistent(hipStream_t s, const PcgBuffers& B, void* barrier);
 // Host view of the scalar state.

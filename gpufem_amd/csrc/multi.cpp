@@ -62,7 +62,7 @@ int run_multi(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, const double* B
     }
   }
   gf_multi_ws* W = P->multi;
-  MultiBuffers B;
+  MultiBuffers B{};
   B.n = n;
   B.S = S;
   B.ncols = ncols;
@@ -74,6 +74,7 @@ int run_multi(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, const double* B
   B.col = is_bsell_variant(P->variant) ? o.col : m->d_col;
   B.bsell_d = is_bsell_variant(P->variant) ? m->dim : 0;
   B.bs = o.bs;
+  B.spmm_lds = multi_spmm_lds(ctx->lds_per_cu);
   B.rownnz = m->d_rownnz;
   B.dinv = opts.precond == GF_PRECOND_JACOBI ? m->d_dinv : nullptr;
   B.b = W->b;

@@ -99,14 +99,15 @@ __global__ __launch_bounds__(kNT) void k_b_init_finish(MultiBuffers B, int nb, i
 }
 
 // q = K p and the per-column partials of p.q
-template <int S, int D>
+template <int S, int D, int NB = 2>
 __global__ __launch_bounds__(kNT) void k_b_spmm_pq_bs(MultiBuffers B) {
   const MultiState<S>* st = static_cast<const MultiState<S>*>(B.st);
   if (st->done) return;
   __shared__ double sh[kNT * kC];
   double pq[kC] = {};
-  for_chunk_node_rows<S, D>(spmm_args(B), B.val, B.p, [&](int32_t R, int c0,
-                                                          const double (*acc)[kC]) {
+  for_chunk_node_rows<S, D, double, double, false, NB>(spmm_args(B), B.val, B.p,
+                                                       [&](int32_t R, int c0,
+                                                           const double (*acc)[kC]) {
 #pragma unroll
     for (int r = 0; r < D; ++r) {
       const int64_t o = (int64_t(D) * R + r) * S + c0;
@@ -323,8 +324,10 @@ inline int grid_for(int64_t work, int threads, int cap) {
 template <int S>
 hipError_t launch_spmm_pq(hipStream_t s, const MultiBuffers& B) {
   const dim3 g(B.nblocks), t(kNT);
-  if (B.bsell_d == 2) hipLaunchKernelGGL((k_b_spmm_pq_bs<S, 2>), g, t, 0, s, B);
-  else if (B.bsell_d == 3) hipLaunchKernelGGL((k_b_spmm_pq_bs<S, 3>), g, t, 0, s, B);
+  // padded LDS: two workgroups per CU keep each XCD's rows in flight (and the
+  // 32-wide rows of p they gather) closer to what its L2 holds
+  if (B.bsell_d == 2) hipLaunchKernelGGL((k_b_spmm_pq_bs<S, 2>), g, t, B.spmm_lds, s, B);
+  else if (B.bsell_d == 3) hipLaunchKernelGGL((k_b_spmm_pq_bs<S, 3>), g, t, B.spmm_lds, s, B);
   else hipLaunchKernelGGL(k_b_spmm_pq<S>, g, t, 0, s, B);
   return hipGetLastError();
 }
@@ -410,6 +413,12 @@ size_t multi_state_bytes(int S) {
     case 32: return sizeof(MultiState<32>);
     default: return sizeof(MultiState<64>);
   }
+}
+
+int multi_spmm_lds(int lds_per_cu) {
+  // k_b_spmm_pq_bs holds kNT * kC doubles of LDS; 512 bytes of allocation slack
+  const int pad = lds_per_cu / 2 - int(kNT * kC * sizeof(double)) - 512;
+  return pad > 0 ? pad : 0;
 }
 
 int multi_blocks(int64_t n, int S) {

@@ -200,7 +200,9 @@ __device__ inline void ell_row_chunk(const SpmmArgs& A, const VT* __restrict__ v
 // DIFF: the values are the difference form of the FP32 copy (see bsell_row):
 // off-diagonal blocks multiply x_J - x_R, the diagonal slot (node column R)
 // holds the block row sum and multiplies x_R.
-template <int S, int D, class T, class VT, bool DIFF = false, class F>
+// NB: blocks loaded per step (their loads in flight together); the blocks
+// are added in slot order whatever NB is.
+template <int S, int D, class T, class VT, bool DIFF = false, int NB = 2, class F>
 __device__ inline void for_chunk_node_rows(const SpmmArgs& A, const VT* __restrict__ val,
                                            const T* __restrict__ xin, F&& f) {
   int g, c0;
@@ -235,11 +237,11 @@ __device__ inline void for_chunk_node_rows(const SpmmArgs& A, const VT* __restri
       }
     };
     int32_t jb = 0;
-    for (; jb + 2 <= cnt; jb += 2) {
-      T v[2][D * D], xv[2][D][kC];
-      int64_t cu[2];
+    for (; jb + NB <= cnt; jb += NB) {
+      T v[NB][D * D], xv[NB][D][kC];
+      int64_t cu[NB];
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
+      for (int u = 0; u < NB; ++u) {
         const int64_t b = b0 + jb + u;
         const int64_t c = int64_t(A.col[b * 64 + lp]) * D;
         cu[u] = c;
@@ -249,7 +251,7 @@ __device__ inline void for_chunk_node_rows(const SpmmArgs& A, const VT* __restri
         for (int e = 0; e < D; ++e) load_chunk(xin + (c + e) * S + c0, xv[u][e]);
       }
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
+      for (int u = 0; u < NB; ++u) {
         rel(cu[u], xv[u]);
 #pragma unroll
         for (int r = 0; r < D; ++r)
@@ -259,7 +261,7 @@ __device__ inline void for_chunk_node_rows(const SpmmArgs& A, const VT* __restri
             for (int j = 0; j < kC; ++j) acc[r][j] += v[u][r * D + e] * xv[u][e][j];
       }
     }
-    if (jb < cnt) {
+    for (; jb < cnt; ++jb) {
       const int64_t b = b0 + jb;
       const int64_t c = int64_t(A.col[b * 64 + lp]) * D;
       T v[D * D], xv[D][kC];

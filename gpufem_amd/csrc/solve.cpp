@@ -684,6 +684,36 @@ int gf_profile_spmv(gf_ctx* ctx, gf_ell* m, int32_t nrep, double* ms_avg) {
   return GF_OK;
 }
 
+int gf_profile_barrier(gf_ctx* ctx, gf_ell* m, int32_t nbar, double* us_per_barrier,
+                       int32_t* nblocks) {
+  if (!ctx || !m || nbar < 1 || !us_per_barrier || !nblocks)
+    return set_error(ctx, GF_ERR_ARG, "gf_profile_barrier: bad argument");
+  GF_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  int st = ensure_plan(ctx, m);
+  if (st) return st;
+  gf_solve_plan* P = m->plan;
+  if (P->variant < 0 && (st = tune_spmv(ctx, m, -1, nullptr, nullptr))) return st;
+  const int nb = persistent_blocks(P->variant, P->num_cu);
+  if (nb < 1)
+    return set_error(ctx, GF_ERR_UNSUPPORTED, "gf_profile_barrier: no persistent grid for this matrix");
+  float ms[2] = {0.f, 0.f};
+  const int counts[2] = {0, nbar};
+  for (int k = 0; k < 2; ++k) {
+    GF_HIP_CHECK(ctx, launch_barrier_probe(ctx->stream, P->barrier, nb, 1));  // warm
+    GF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
+    GF_HIP_CHECK(ctx, launch_barrier_probe(ctx->stream, P->barrier, nb, counts[k]));
+    GF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
+    GF_HIP_CHECK(ctx, hipEventSynchronize(ctx->ev_stop));
+    GF_HIP_CHECK(ctx, hipEventElapsedTime(&ms[k], ctx->ev_start, ctx->ev_stop));
+  }
+  int aborted = 0;
+  GF_HIP_CHECK(ctx, hipError_t(barrier_aborted(ctx->stream, P->barrier, &aborted)));
+  if (aborted) return set_error(ctx, GF_ERR_HIP, "gf_profile_barrier: a barrier wait timed out");
+  *us_per_barrier = 1e3 * double(ms[1] - ms[0]) / nbar;
+  *nblocks = nb;
+  return GF_OK;
+}
+
 int gf_profile_pcg(gf_ctx* ctx, gf_ell* m, const double* b, int32_t iters, double* ms_spmv,
                    double* ms_iteration) {
   if (!ctx || !m || !b || iters < 1 || !ms_spmv || !ms_iteration)

@@ -385,6 +385,14 @@ __global__ __launch_bounds__(kNT) void k_pcg_persistent(PcgBuffers B, GridBarrie
   }
 }
 
+// nbar grid barriers and nothing else: the synchronisation cost of the
+// persistent schedule (three per iteration) on its own grid
+__global__ __launch_bounds__(kNT) void k_barrier_probe(GridBarrier* bar, int nbar) {
+  __shared__ int s_flag;
+  for (int k = 0; k < nbar; ++k)
+    if (!grid_barrier(bar, &s_flag)) break;
+}
+
 __global__ void k_barrier_reset(GridBarrier* bar) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     bar->count = 0;
@@ -605,6 +613,24 @@ hipError_t launch_pcg_persistent(hipStream_t s, const PcgBuffers& B, void* barri
     case kBsell3G: return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent<kBsell3G>), g, blk, args, 0, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_barrier_probe(hipStream_t s, void* barrier, int nblocks, int nbar) {
+  GridBarrier* bar = static_cast<GridBarrier*>(barrier);
+  hipLaunchKernelGGL(k_barrier_reset, dim3(1), dim3(64), 0, s, bar);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  void* args[] = {&bar, &nbar};
+  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_barrier_probe), dim3(nblocks),
+                                    dim3(kNT), args, 0, s);
+}
+
+int barrier_aborted(hipStream_t s, const void* barrier, int* aborted) {
+  GridBarrier h;
+  hipError_t e = hipMemcpyAsync(&h, barrier, sizeof(h), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  *aborted = e == hipSuccess ? h.abort : 1;
+  return int(e);
 }
 
 hipError_t read_pcg_state(hipStream_t s, const PcgBuffers& B, PcgHostState* h) {

@@ -337,6 +337,14 @@ int gf_profile_spmv(gf_ctx* ctx, gf_ell* m, int32_t nrep, double* ms_avg);
  * counterpart; the same per-kernel event timing as gf_profile_spmv. */
 int gf_profile_pcg(gf_ctx* ctx, gf_ell* m, const double* b, int32_t iters, double* ms_spmv,
                    double* ms_iteration);
+/* Cost of one grid barrier of the persistent schedule (GF_SCHED_PERSISTENT,
+ * three barriers per iteration): nbar barriers with no work between them on
+ * the persistent kernel's co-resident grid for the matrix's SpMV variant
+ * (*nblocks workgroups), timed with HIP events, minus an empty launch;
+ * *us_per_barrier in microseconds.  No reference counterpart (the reference
+ * ends every CG step with a host synchronisation, ELLmatrix.cc:315-459). */
+int gf_profile_barrier(gf_ctx* ctx, gf_ell* m, int32_t nbar, double* us_per_barrier,
+                       int32_t* nblocks);
 /* Bandwidth probe: nrep launches of a plain streaming copy kernel of `bytes`
  * (multiple of 16; 16-byte aligned device pointers) on the context's stream;
  * *ms_avg = average launch time.  Measurement aid with no reference

@@ -51,7 +51,7 @@ SIGS = {
     "orc_apply_elimination": (None, [_EP, C.c_int, _I64, _P]),
     "orc_dot": (_D, [_I64, _P, _P, C.c_int]),
     "orc_cg": (C.c_int, [_EP, _P, _P, C.c_int, _D, C.c_int, _P]),
-    "orc_pcg_jacobi": (C.c_int, [_EP, _P, _P, C.c_int, _D, C.c_int, C.c_int, _P, _P]),
+    "orc_pcg_jacobi": (C.c_int, [_EP, _P, _P, C.c_int, _D, C.c_int, C.c_int, _P, _P, C.c_int]),
 }
 
 _lib = None
@@ -278,11 +278,12 @@ def cg(K, b, maxit, eps, nthreads=None):
     return x, it, secs.value
 
 
-def pcg_jacobi(K, b, rtol=1e-8, maxit=100000, x0=None, nthreads=None):
+def pcg_jacobi(K, b, rtol=1e-8, maxit=100000, x0=None, nthreads=None, progress_every=0):
     b = np.ascontiguousarray(b, dtype=np.float64)
     x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
     rel = C.c_double()
     secs = C.c_double()
     it = load().orc_pcg_jacobi(K.m, _p(b), _p(x), maxit, rtol, 0 if x0 is None else 1,
-                               nthreads or default_threads(), C.byref(rel), C.byref(secs))
+                               nthreads or default_threads(), C.byref(rel), C.byref(secs),
+                               int(progress_every))
     return x, it, rel.value, secs.value

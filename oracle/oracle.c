@@ -44,6 +44,7 @@
 
 #include <math.h>
 #include <omp.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -856,7 +857,8 @@ int orc_cg(const orc_ell* K, const double* b, double* x, int maxit, double eps,
 }
 
 int orc_pcg_jacobi(const orc_ell* K, const double* b, double* x, int maxit, double rtol,
-                   int use_x0, int nthreads, double* true_relres, double* seconds) {
+                   int use_x0, int nthreads, double* true_relres, double* seconds,
+                   int progress_every) {
   /* Saad, Iterative Methods for Sparse Linear Systems, 2nd ed., Alg. 9.1,
    * with M = diag(K).  The vector passes mirror CPU_CG's separate BLAS-1
    * loops (src/SPRmatrix/SPRmatrix.cc:432-451). */
@@ -905,6 +907,12 @@ int orc_pcg_jacobi(const orc_ell* K, const double* b, double* x, int maxit, doub
 #pragma omp parallel for num_threads(nthreads) schedule(static)
     for (int64_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
     ++it;
+    if (progress_every > 0 && it % progress_every == 0) {
+      /* long host solves report as they go */
+      fprintf(stderr, "[oracle] pcg iteration %d, %.1f s, residual %.3e of |b|\n", it,
+              omp_get_wtime() - t0, bnorm > 0 ? sqrt(rr) / bnorm : sqrt(rr));
+      fflush(stderr);
+    }
   }
   if (seconds) *seconds = omp_get_wtime() - t0;
   if (true_relres) {

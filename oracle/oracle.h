@@ -147,9 +147,11 @@ int orc_cg(const orc_ell* K, const double* b, double* x, int maxit, double eps,
 /* Jacobi-preconditioned CG (Saad, Iterative Methods, Alg. 9.1).  If
  * use_x0 is 0, x starts at zero.  Stops when ||r||_2 <= rtol ||b||_2
  * (recursive residual) or after maxit iterations.  Returns iterations;
- * *true_relres gets ||b - K x|| / ||b|| at exit. */
+ * *true_relres gets ||b - K x|| / ||b|| at exit.  progress_every > 0: a line
+ * on stderr every that many iterations. */
 int orc_pcg_jacobi(const orc_ell* K, const double* b, double* x, int maxit, double rtol,
-                   int use_x0, int nthreads, double* true_relres, double* seconds);
+                   int use_x0, int nthreads, double* true_relres, double* seconds,
+                   int progress_every);
 
 #ifdef __cplusplus
 }

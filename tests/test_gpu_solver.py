@@ -390,6 +390,19 @@ def test_persistent_matches_kernel_schedule(gpu_ctx, et, dims, variant):
     assert np.array_equal(x1, x2)
 
 
+def test_profile_barrier(gpu_ctx):
+    """Grid barriers of the persistent schedule on its own co-resident grid:
+    a positive cost per barrier, a grid of whole XCD rounds, and repeated
+    calls reset the barrier (no wait runs into its spin limit)."""
+    p, K = assembled(gpu_ctx, gf.GF_BRICK8, 9, 8, 7)
+    gpu_ctx.tune(K, 10)
+    for nbar in (200, 50):
+        us, nb = gpu_ctx.profile_barrier(K, nbar)
+        assert 0.0 < us < 1000.0 and nb > 0 and nb % 8 == 0
+    with pytest.raises(gf.GpufemError):
+        gpu_ctx.profile_barrier(K, 0)
+
+
 def test_profile_pcg_times_the_solver_spmv(gpu_ctx):
     """Events around each SpMV of real PCG iterations: the kernel's share of
     an iteration; a system solved exactly before the count is an error."""
