@@ -458,6 +458,7 @@ int gf_ell_download(gf_ctx* ctx, const gf_ell* m, double* val, int32_t* col,
   int st = select_device(ctx);
   if (st) return st;
   const size_t slots = size_t(m->n) * size_t(m->width);
+  if (val && (st = ensure_ell_vals(ctx, const_cast<gf_ell*>(m)))) return st;
   GF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   if (val) GF_HIP_CHECK(ctx, hipMemcpy(val, m->d_val, slots * 8, hipMemcpyDeviceToHost));
   if (col) GF_HIP_CHECK(ctx, hipMemcpy(col, m->d_col, slots * 4, hipMemcpyDeviceToHost));
@@ -545,6 +546,11 @@ static int assemble_with(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int varian
   // the ELL values, the colored scatters are followed by the copy
   const bool bsell = m->d_bval && m->bval_version != 0;
   a.bval = bsell && (variant == 3 || variant == 4) ? m->d_bval : nullptr;
+  // with the copy current the gather writes only the diagonal of the ELL
+  // values (what the penalty and the Jacobi preconditioner read); the other
+  // slots are materialised from the copy when something reads them
+  a.diag = m->d_diag;
+  a.ell_diag_only = bsell && variant == 3;
   a.bsptr = m->d_bsptr;
   a.binv = m->d_binv;
   a.col = m->d_col;
@@ -571,6 +577,7 @@ static int assemble_with(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int varian
   }
   double ms = 0;
   if ((st = record_stop(ctx, &ms))) return st;
+  m->ell_stale = a.ell_diag_only != 0;
   int bad = 0;
   GF_HIP_CHECK(ctx, hipMemcpy(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost));
   m->dinv_valid = false;
@@ -726,6 +733,7 @@ int gf_apply_elimination(gf_ctx* ctx, gf_ell* m, const int32_t* dofs, int64_t nd
     return set_error(ctx, GF_ERR_ARG, "gf_apply_elimination: bad argument");
   int st = select_device(ctx);
   if (st) return st;
+  if ((st = ensure_ell_vals(ctx, m))) return st;
   int32_t* d_dofs = nullptr;
   if ((st = upload_dofs(ctx, "gf_apply_elimination", dofs, nd, m->n, &d_dofs))) return st;
   uint8_t* d_fixed = nullptr;

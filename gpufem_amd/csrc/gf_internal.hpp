@@ -85,6 +85,12 @@ struct gf_ell {
   // node rows where block-sliced windows restart (a grouped batch: the first
   // node of every mesh, then the node count), and the first slice of each
   std::vector<int64_t> bsell_breaks, bsell_break_slice;
+  // The node gather, once the block-sliced copy exists, writes that copy and
+  // only the diagonal of the ELL values: then ell_stale is set, the copy is
+  // current (bval_version == val_version), d_val holds a current diagonal
+  // and stale off-diagonal slots.  ensure_ell_vals materialises d_val from
+  // the copy for every reader of the other slots.
+  bool ell_stale = false;
   int32_t* d_bcol = nullptr;
   int16_t* d_bcoff = nullptr;  // 16-bit node-column offsets (null: some did not fit)
   double* d_bval = nullptr;

@@ -325,6 +325,8 @@ int gf_group_apply_penalty(gf_ctx* ctx, gf_group* g, const int32_t* dofs, const 
     i = j;
   }
   GF_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  int st = ensure_ell_vals(ctx, g->ell);
+  if (st) return st;
   int64_t* d_d = nullptr;
   int32_t* d_mc = nullptr;
   const size_t nd = ud.size();
@@ -364,6 +366,9 @@ int gf_group_pcg(gf_ctx* ctx, gf_group* g, const double* b, double* x, const gf_
   if (m->plan->variant < 0 && (st = tune_spmv(ctx, m, -1, nullptr, nullptr))) return st;
   if ((st = prepare_jacobi(ctx, m))) return st;
   if ((st = prepare_variant(ctx, m, group_variant(m)))) return st;
+  // the ELL values unless the grouped kernels read the block-sliced copy
+  if (!(is_bsell_variant(group_variant(m)) && g->d_sg_ptr) && (st = ensure_ell_vals(ctx, m)))
+    return st;
   if (is_bsell_variant(group_variant(m)) && (st = ensure_slice_groups(ctx, g))) return st;
   const int64_t n = m->n;
   const size_t vb = size_t(n) * sizeof(double);

@@ -443,6 +443,9 @@ struct GArgs {
   double* bval;
   const int64_t* bsptr;
   const int32_t* binv;
+  // ell_diag_only: of the ELL values only the diagonal (slot diag[row])
+  const int32_t* diag;
+  int ell_diag_only;
 };
 
 // A unit is RPT rows of one node (UPN units per node).  LPU lanes share a
@@ -647,11 +650,13 @@ __global__ __launch_bounds__(64) void k_assemble_gather(GArgs ga) {
         const int32_t ns = ga.rownnz[v * DIM] / DIM;
         for (int s = lj; s < ns; s += LPU) {
           const double* sa = s_acc + int64_t(s) * DD * UPW + lu;
+          if (!ga.ell_diag_only) {
 #pragma unroll
-          for (int j = 0; j < DIM; ++j)
+            for (int j = 0; j < DIM; ++j)
 #pragma unroll
-            for (int r = 0; r < DIM; ++r)
-              a.val[v * DIM + r + (int64_t(s) * DIM + j) * a.n] = sa[(r * DIM + j) * UPW];
+              for (int r = 0; r < DIM; ++r)
+                a.val[v * DIM + r + (int64_t(s) * DIM + j) * a.n] = sa[(r * DIM + j) * UPW];
+          }
           if (ga.bval) {
             typedef double P2 __attribute__((ext_vector_type(2)));
             double* blk = ga.bval + (bb0 + s) * DD * 64;
@@ -665,6 +670,16 @@ __global__ __launch_bounds__(64) void k_assemble_gather(GArgs ga) {
             if constexpr (DD & 1) blk[(DD - 1) * 64 + bl] = sa[(DD - 1) * UPW];
           }
         }
+        if (ga.ell_diag_only && lj == 0) {
+#pragma unroll
+          for (int r = 0; r < DIM; ++r) {
+            const int64_t row = v * DIM + r;
+            const int32_t ds = ga.diag[row];
+            if (ds >= 0)
+              a.val[row + int64_t(ds) * a.n] =
+                  s_acc[(int64_t(ds / DIM) * DD + r * DIM + ds % DIM) * UPW + lu];
+          }
+        }
       } else {
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
@@ -674,9 +689,13 @@ __global__ __launch_bounds__(64) void k_assemble_gather(GArgs ga) {
 #pragma unroll
             for (int j = 0; j < DIM; ++j) {
               const double kv = s_acc[((s * RPT + r) * DIM + j) * UPW + lu];
-              a.val[row + (int64_t(s) * DIM + j) * a.n] = kv;
+              if (!ga.ell_diag_only) a.val[row + (int64_t(s) * DIM + j) * a.n] = kv;
               if (ga.bval) ga.bval[bsell_vidx(DIM * DIM, bb0 + s, (r0 + r) * DIM + j, bl)] = kv;
             }
+          const int32_t ds = ga.diag[row];
+          if (ga.ell_diag_only && lj == 0 && ds >= 0)
+            a.val[row + int64_t(ds) * a.n] =
+                s_acc[((int64_t(ds / DIM) * RPT + r) * DIM + ds % DIM) * UPW + lu];
         }
       }
     }
@@ -1250,6 +1269,8 @@ hipError_t launch_assembly_gather(hipStream_t s, const AssemblyArgs& a) {
   g.bval = a.bval;
   g.bsptr = a.bsptr;
   g.binv = a.binv;
+  g.diag = a.diag;
+  g.ell_diag_only = a.bval && a.ell_diag_only ? 1 : 0;
   const bool g3 = a.ngauss == 3;
   switch (a.type) {
     case GF_Q4: return g3 ? launch_gather2<4, 9, 9>(s, g) : launch_gather2<4, 4, 9>(s, g);

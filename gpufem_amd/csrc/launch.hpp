@@ -43,6 +43,10 @@ struct AssemblyArgs {
   double* bval;
   const int64_t* bsptr;
   const int32_t* binv;
+  // node gather with the copy: ell_diag_only = 1 writes only the diagonal
+  // (slot diag[row]) of the ELL values
+  const int32_t* diag;
+  int ell_diag_only;
   // two-phase assembly: ELL columns and the element count
   const int32_t* col;
   int64_t nelem;
@@ -160,6 +164,9 @@ hipError_t launch_bsell_offsets(hipStream_t s, const BsellView& bs, const int32_
                                 int16_t* coff, int* overflow);
 hipError_t launch_bsell_vals(hipStream_t s, const BsellView& bs, int d, int64_t n,
                              const double* val, const int32_t* rownnz, double* bval);
+// ELL values of the live slots from the block-sliced copy (the inverse)
+hipError_t launch_bsell_to_ell(hipStream_t s, const BsellView& bs, int d, int64_t n,
+                               const double* bval, const int32_t* rownnz, double* val);
 constexpr int kMaxPcgBlocks = 2048;
 // SpMV workgroups of the slice-group variants (one partial p.q each)
 constexpr int kMaxSpmvBlocks = 65536;

@@ -105,6 +105,12 @@ extern "C" int gf_pcg_noalloc(gf_ctx* ctx, gf_ell* m, const double* b, double* x
                                               : true;
     if (current) variant = v;
   }
+  // the ELL values of every slot for the ELL and node-column kernels (a
+  // kernel launch, no allocation)
+  if (!is_bsell_variant(variant)) {
+    const int est = ensure_ell_vals(ctx, m);
+    if (est) return est;
+  }
   const SpmvOperands o = spmv_operands(m, variant);
   PcgBuffers B;
   B.n = n;

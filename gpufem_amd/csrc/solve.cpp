@@ -238,11 +238,22 @@ int ensure_bsell(gf_ctx* ctx, gf_ell* m) {
   return GF_OK;
 }
 
+int ensure_ell_vals(gf_ctx* ctx, gf_ell* m) {
+  if (!m->ell_stale) return GF_OK;
+  GF_HIP_CHECK(ctx, launch_bsell_to_ell(ctx->stream, bsell_view(m), m->dim, m->n, m->d_bval,
+                                        m->d_rownnz, m->d_val));
+  m->ell_stale = false;
+  return GF_OK;
+}
+
 // Values of the copy follow the ELL values (assembly and penalty bump the version).
 int ensure_bsell_vals(gf_ctx* ctx, gf_ell* m) {
   int st = ensure_bsell(ctx, m);
   if (st) return st;
   if (m->bval_version != m->val_version) {
+    // stale ELL values come with a current copy; anything else is a bug
+    if (m->ell_stale)
+      return set_error(ctx, GF_ERR_HIP, "block-sliced copy and ELL values both stale");
     GF_HIP_CHECK(ctx, launch_bsell_vals(ctx->stream, bsell_view(m), m->dim, m->n, m->d_val,
                                         m->d_rownnz, m->d_bval));
     m->bval_version = m->val_version;
@@ -251,8 +262,10 @@ int ensure_bsell_vals(gf_ctx* ctx, gf_ell* m) {
 }
 
 int prepare_variant(gf_ctx* ctx, gf_ell* m, int v) {
-  if (is_node_variant(v)) return ensure_coln(ctx, m);
   if (is_bsell_variant(v)) return ensure_bsell_vals(ctx, m);
+  int st = ensure_ell_vals(ctx, m);
+  if (st) return st;
+  if (is_node_variant(v)) return ensure_coln(ctx, m);
   return GF_OK;
 }
 

@@ -49,6 +49,9 @@ SpmvOperands spmv_operands(const gf_ell* m, int variant);
 BsellView bsell_view(const gf_ell* m);
 // builds what the variant reads and refreshes copies of stale values
 int prepare_variant(gf_ctx* ctx, gf_ell* m, int variant);
+// the ELL values of every slot current (from the block-sliced copy when the
+// node gather left them stale)
+int ensure_ell_vals(gf_ctx* ctx, gf_ell* m);
 // the node-block / block-sliced (swept) variant of the matrix's dimension, or -1
 int node_variant(const gf_ell* m);
 int bsell_variant(const gf_ell* m);

@@ -127,6 +127,25 @@ __global__ void k_bsell_vals(BsellView bs, int d, int64_t n, const double* __res
   }
 }
 
+// The inverse: the ELL values of the live slots from the block-sliced copy
+// (padding slots of the ELL rows keep their zeros).
+__global__ void k_bsell_to_ell(BsellView bs, int d, int64_t n, const double* __restrict__ bval,
+                               const int32_t* __restrict__ rownnz, double* __restrict__ val) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < bs.nslices * 64;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t sl = t >> 6;
+    const int lane = int(t & 63);
+    const int32_t R = bs.perm[t];
+    if (R < 0) continue;
+    const int64_t b0 = bs.sptr[sl];
+    const int32_t cnt = rownnz[int64_t(d) * R] / d;
+    for (int64_t j = 0; j < cnt; ++j)
+      for (int e = 0; e < d; ++e)
+        for (int r = 0; r < d; ++r)
+          val[int64_t(d) * R + r + (j * d + e) * n] = bval[bsell_vidx(d * d, b0 + j, r * d + e, lane)];
+  }
+}
+
 __global__ void k_build_coln(int64_t n, int32_t nbw, int d, const int32_t* col, int32_t* coln) {
   const int64_t total = n * nbw;
   for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
@@ -495,6 +514,13 @@ hipError_t launch_bsell_vals(hipStream_t s, const BsellView& bs, int d, int64_t 
                              const double* val, const int32_t* rownnz, double* bval) {
   hipLaunchKernelGGL(k_bsell_vals, dim3(grid_for(bs.nslices * 64, kNT, 8192)), dim3(kNT), 0, s, bs,
                      d, n, val, rownnz, bval);
+  return hipGetLastError();
+}
+
+hipError_t launch_bsell_to_ell(hipStream_t s, const BsellView& bs, int d, int64_t n,
+                               const double* bval, const int32_t* rownnz, double* val) {
+  hipLaunchKernelGGL(k_bsell_to_ell, dim3(grid_for(bs.nslices * 64, kNT, 8192)), dim3(kNT), 0, s,
+                     bs, d, n, bval, rownnz, val);
   return hipGetLastError();
 }
 

@@ -47,6 +47,44 @@ def test_gather_writes_block_sliced_copy(gpu_ctx, t, dims, bsell):
     assert np.abs(y0 - yo).max() <= 1e-12 * np.abs(yo).max()
 
 
+@pytest.mark.parametrize("t,dims,bsell", [(gf.GF_Q8, (20, 10), 9), (gf.GF_BRICK8, (7, 6, 5), 10),
+                                          (gf.GF_BRICK20, (4, 3, 3), 10)])
+def test_stale_ell_values_materialise(gpu_ctx, t, dims, bsell):
+    """Once the copy exists the node gather writes it and only the diagonal
+    of the ELL values; every reader of the other slots (download, the
+    elimination, the ELL SpMV kernels) gets them materialised from the copy:
+    the results are bitwise those of a matrix whose gather wrote every ELL
+    value (no copy built), penalty and elimination included."""
+    p = P.structured(t, *dims)
+    E = 210000.0 * np.exp(0.4 * np.random.default_rng(3).normal(size=p.grid.nelem))
+    mesh = gpu_ctx.upload_mesh(p.elem_type, p.ngauss, p.conn, p.coords, p.E, p.nu, elem_E=E)
+    K = gpu_ctx.symbolic(mesh)      # with the copy: ELL values left stale
+    K2 = gpu_ctx.symbolic(mesh)     # without a copy: every ELL value written
+    K3 = gpu_ctx.symbolic(mesh)     # copy built from fully written ELL values
+    for M in (K, K2, K3):
+        gpu_ctx.assemble_tune(mesh, M, 3)
+    gpu_ctx.tune(K, bsell)
+    gpu_ctx.tune(K2, 0)
+    for M in (K, K2, K3):
+        gpu_ctx.assemble(mesh, M)
+        gpu_ctx.apply_penalty(M, p.fixed_dofs)
+    gpu_ctx.tune(K3, bsell)
+    x1, s1 = gpu_ctx.pcg(K, p.f)                # reads the copy; ELL stays stale
+    x3, s3 = gpu_ctx.pcg(K3, p.f)
+    assert s1.iterations == s3.iterations and np.array_equal(x1, x3)
+    v1, c1, _ = gpu_ctx.download(K)             # materialised
+    v2, c2, _ = gpu_ctx.download(K2)
+    assert np.array_equal(v1, v2) and np.array_equal(c1, c2)
+    # elimination after a stale assembly
+    dofs = np.asarray(p.fixed_dofs, dtype=np.int32)
+    for M in (K, K2):
+        gpu_ctx.assemble(mesh, M)
+        gpu_ctx.apply_elimination(M, dofs)
+    v1, _, _ = gpu_ctx.download(K)
+    v2, _, _ = gpu_ctx.download(K2)
+    assert np.array_equal(v1, v2)
+
+
 def test_wide_node_columns_match_thread_per_row(gpu_ctx):
     """A Q4 plate of 33,153 nodes numbered at random: neighbouring nodes lie
     more than 32,767 apart, so the block-sliced kernels read 32-bit node
