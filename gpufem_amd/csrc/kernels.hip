@@ -460,8 +460,9 @@ struct GArgs {
 // elastic combinations once per element; the element's nodes b are taken BCH
 // at a time (the Gauss-point geometry is recomputed per chunk), which bounds
 // the accumulator registers.  With RPT < DIM (one row per unit) the row's
-// entries are accumulated directly and BCH = NEN.
-template <int DIM, int NEN, int NGP, int MAXS, int RPT, int LPU, int UPW, int BCH>
+// entries are accumulated directly and BCH = NEN.  GPU: unroll factor of the
+// Gauss-point loop.
+template <int DIM, int NEN, int NGP, int MAXS, int RPT, int LPU, int UPW, int BCH, int GPU = 1>
 __global__ __launch_bounds__(64) void k_assemble_gather(GArgs ga) {
   static_assert(LPU * UPW <= 64, "a workgroup is one wavefront");
   static_assert(NEN % BCH == 0 && (RPT == DIM || BCH == NEN), "neighbour chunks");
@@ -511,6 +512,7 @@ __global__ __launch_bounds__(64) void k_assemble_gather(GArgs ga) {
           for (int r = 0; r < RPT; ++r)
 #pragma unroll
             for (int j = 0; j < DIM; ++j) kl[b][r][j] = 0.0;
+#pragma unroll(GPU)
         for (int g = 0; g < NGP; ++g) {
           const double* dN = c_tab + a.tab + g * DIM * NEN;
           double J[DIM][DIM];
@@ -1128,13 +1130,11 @@ __global__ __launch_bounds__(256) void k_assemble_pairs(PArgs pa) {
 }
 
 namespace {
-template <int DIM, int NEN, int NGP, int MAXS, int RPT, int LPU, int UPW, int BCH>
+template <int DIM, int NEN, int NGP, int MAXS, int RPT, int LPU, int UPW, int BCH, int GPU = 1>
 hipError_t launch_gather_cfg(hipStream_t s, const GArgs& g) {
   const int64_t units = g.nnodes * (DIM / RPT);
   if (units == 0) return hipSuccess;
-  // one tile of UPW units per workgroup: a wavefront ends after its stores
-  // instead of waiting for them before the loads of a next tile
-  hipLaunchKernelGGL((k_assemble_gather<DIM, NEN, NGP, MAXS, RPT, LPU, UPW, BCH>),
+  hipLaunchKernelGGL((k_assemble_gather<DIM, NEN, NGP, MAXS, RPT, LPU, UPW, BCH, GPU>),
                      dim3(grid_for(units, UPW, 1 << 30)), dim3(64), 0, s, g);
   return hipGetLastError();
 }
@@ -1144,7 +1144,9 @@ hipError_t launch_gather_cfg(hipStream_t s, const GArgs& g) {
 //  2D: node units (both rows), 2 lanes per node; Q8 1024x512 0.62 ms (one lane
 //      per node: 0.86 ms, 43 KB of LDS per workgroup, 1 wavefront per SIMD).
 //  Brick8: node units, 4 lanes per node, 16 nodes per workgroup, neighbours in
-//      chunks of 4 (223 VGPRs): 128^3 4.6 ms; row units (one row per unit,
+//      chunks of 4, Gauss-point loop unrolled by 2 (the table loads of two
+//      points batched: 4.38 -> 4.18 ms; by 4: 4.75 ms; the 2D and Brick20
+//      kernels gain nothing or lose): 128^3 4.2 ms; row units (one row per unit,
 //      each element's geometry once per row): 6.3-6.6 ms with 2 lanes per row,
 //      9.7 ms with one lane per row.
 //  Brick20: row units, 4 lanes per row (256 VGPRs + AGPRs): 48^3 8.1-8.7 ms;
@@ -1276,8 +1278,8 @@ hipError_t launch_assembly_gather(hipStream_t s, const AssemblyArgs& a) {
     case GF_Q4: return g3 ? launch_gather2<4, 9, 9>(s, g) : launch_gather2<4, 4, 9>(s, g);
     case GF_Q8: return g3 ? launch_gather2<8, 9, 21>(s, g) : launch_gather2<8, 4, 21>(s, g);
     case GF_BRICK8:
-      return g3 ? launch_gather_cfg<3, 8, 27, 27, 3, 4, 16, 4>(s, g)
-                : launch_gather_cfg<3, 8, 8, 27, 3, 4, 16, 4>(s, g);
+      return g3 ? launch_gather_cfg<3, 8, 27, 27, 3, 4, 16, 4, 2>(s, g)
+                : launch_gather_cfg<3, 8, 8, 27, 3, 4, 16, 4, 2>(s, g);
     case GF_BRICK20:
       return g3 ? launch_gather_cfg<3, 20, 27, 81, 1, 4, 16, 20>(s, g)
                 : launch_gather_cfg<3, 20, 8, 81, 1, 4, 16, 20>(s, g);
