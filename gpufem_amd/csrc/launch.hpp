@@ -235,5 +235,7 @@ hipError_t read_pcg_state(hipStream_t s, const PcgBuffers& B, PcgHostState* h);
 hipError_t launch_spmv_pq_repeat(hipStream_t s, const PcgBuffers& B, int nrep);
 // true residual ||b - K x|| / ||b|| into *out (device), deterministic
 hipError_t launch_true_residual(hipStream_t s, const PcgBuffers& B, double* out_dev);
+// *out_dev = 1/2 x.Kx - b.x (B.q is overwritten with K x)
+hipError_t launch_energy(hipStream_t s, const PcgBuffers& B, double* out_dev);
 
 }  // namespace gf

@@ -60,15 +60,18 @@ constexpr int kStallRefreshes = 3;
 // the Lanczos coefficients of the Jacobi-scaled operator: T_k (diagonal
 // 1/alpha_j + beta_{j-1}/alpha_{j-1}, off-diagonal sqrt(beta_j)/alpha_j) has
 // extreme eigenvalues that approach those of the operator, lambda_min from
-// above.  Once u32 lambda_max(T_k) / lambda_min(T_k) exceeds kKappaU32 the
-// FP32 rounding of the operator and of the vectors is larger than its
-// smallest eigenvalues: FP32 iterations can no longer resolve the slowest
-// error components and the solve continues in FP64 (the same status as a
-// stall).  Checked every kLanczosCheck iterations over the first
-// kLanczosIters.
+// above.  When u32 lambda_max(T_k) / lambda_min(T_k) approaches 1 the FP32
+// rounding of the operator and of the vectors reaches its smallest
+// eigenvalues: FP32 iterations can no longer resolve the slowest error
+// components and the solve continues in FP64 (the same status as a stall).
+// The estimate grows slowly (lambda_min from above), so the hand-over is at
+// kKappaU32 = 0.05: a Q8 1024x512 plate (u32 kappa ~ 12) crosses it after
+// 1,024 iterations instead of 2,304 at 1; the estimates of the Brick8 128^3
+// and Brick20 48^3 cubes stay below 0.018.  Checked every kLanczosCheck
+// iterations over the first kLanczosIters.
 constexpr int kLanczosIters = 8192;
 constexpr int kLanczosCheck = 256;
-constexpr double kKappaU32 = 1.0;
+constexpr double kKappaU32 = 0.05;
 constexpr double kU32 = 5.9604644775390625e-8;  // 2^-24
 // lambda_max / lambda_min of T_k from k CG steps (alpha[j], beta[j] at
 // stride `stride`); 0 when T_k is not positive definite or k < 2

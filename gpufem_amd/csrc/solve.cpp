@@ -620,8 +620,19 @@ int gf_pcg(gf_ctx* ctx, gf_ell* m, const double* b, double* x, int32_t nrhs,
         // (breakdown: re-solve in FP64 from zero) or unable to resolve the
         // remaining error (stall: continue in FP64 from the current x), with
         // the iterations the FP32 part left
-        const int warm = h.status == kMixedStalled ? 1 : 0;
+        int warm = h.status == kMixedStalled ? 1 : 0;
         PcgBuffers Bj = make_buffers(m, m->d_dinv);
+        if (warm) {
+          // the FP32 iterate can carry large errors in the lowest modes (a
+          // small residual, a large energy error): FP64 continues from it only
+          // when its energy functional is below the zero start's
+          GF_HIP_CHECK(ctx, launch_energy(ctx->stream, Bj, P->res + k));
+          double energy = 0.0;
+          GF_HIP_CHECK(ctx, hipMemcpyAsync(&energy, P->res + k, sizeof(double),
+                                           hipMemcpyDeviceToHost, ctx->stream));
+          GF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+          if (!(energy < 0.0)) warm = 0;
+        }
         st = run_pcg(ctx, m, Bj, sched == GF_SCHED_PERSISTENT ? sched : GF_SCHED_KERNELS, 1, warm,
                      opts.rtol, opts.maxit - mixed_it, opts.verify, &h);
         if (st) return st;

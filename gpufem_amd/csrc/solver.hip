@@ -447,6 +447,32 @@ __global__ __launch_bounds__(kNT) void k_resid_finish(PcgBuffers B, int nb, doub
   if (threadIdx.x == 0) out[0] = bb > 0.0 ? sqrt(s / bb) : sqrt(s);
 }
 
+// The energy functional 1/2 x.Kx - b.x (q holds K x): CG's error measure up
+// to a constant, 1/2 ||x - x*||_K^2 - 1/2 ||x*||_K^2, so x beats the zero
+// start when it is negative.
+__global__ __launch_bounds__(kNT) void k_energy_partial(PcgBuffers B) {
+  __shared__ double sh[kNT / 64];
+  double xq = 0.0, xb = 0.0;
+  for_my_rows(B.n, [&](int64_t i) {
+    const double xi = B.x[i];
+    xq += xi * B.q[i];
+    xb += xi * B.b[i];
+  });
+  xq = block_sum(xq, sh);
+  xb = block_sum(xb, sh);
+  if (threadIdx.x == 0) {
+    B.part1[blockIdx.x] = xq;
+    B.part2[blockIdx.x] = xb;
+  }
+}
+
+__global__ __launch_bounds__(kNT) void k_energy_finish(PcgBuffers B, int nb, double* out) {
+  __shared__ double sh[kNT / 64];
+  const double xq = reduce_partials(B.part1, nb, sh);
+  const double xb = reduce_partials(B.part2, nb, sh);
+  if (threadIdx.x == 0) out[0] = 0.5 * xq - xb;
+}
+
 __global__ void k_state_live(PcgState* st) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     st->done = 0;
@@ -682,6 +708,15 @@ hipError_t launch_spmv_pq_repeat(hipStream_t s, const PcgBuffers& B, int nrep) {
     e = hipGetLastError();
   }
   return e;
+}
+
+hipError_t launch_energy(hipStream_t s, const PcgBuffers& B, double* out_dev) {
+  hipError_t e = launch_spmv(s, B.variant, B.n, B.val, B.col, B.rownnz, B.x, B.q, 1, &B.bs);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_energy_partial, dim3(B.nblocks), dim3(kNT), 0, s, B);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_energy_finish, dim3(1), dim3(kNT), 0, s, B, B.nblocks, out_dev);
+  return hipGetLastError();
 }
 
 hipError_t launch_true_residual(hipStream_t s, const PcgBuffers& B, double* out_dev) {
