@@ -253,7 +253,7 @@ def form_grouped(ctx, count=256):
     return r
 
 
-def form_learned(ctx, weights, members=16):
+def form_learned(ctx, weights, members=256):
     from . import learned as L
     if not os.path.exists(weights):
         return {"skipped": "no trained weights at %s" % weights}
@@ -261,7 +261,7 @@ def form_learned(ctx, weights, members=16):
     fam = L.Family()
     ks = list(range(L.HOLDOUT_BASE, L.HOLDOUT_BASE + members))
     _log("learned: %d held-out cantilevers" % members)
-    recs = L.evaluate(ctx, fam, net, c_u, ks, rtol=RTOL, log=_log)
+    recs = L.evaluate(ctx, fam, net, c_u, ks, rtol=RTOL, log=_log, log_every=32)
     s = L.summarize(recs)
     s["workload"] = "brick8_cantilever_64x32x32"
     s["trained_on"] = meta.get("train_members")
