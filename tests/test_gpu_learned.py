@@ -17,6 +17,7 @@ import numpy as np
 import pytest
 
 import gpufem_amd as gf
+from gpufem_amd import forms
 from gpufem_amd import learned as L
 from oracle import nn_ref as R
 
@@ -66,3 +67,15 @@ def test_inference_matches_network_oracle(gpu_ctx, trained):
     ref = R.Net(fam.nx + 1, fam.ny + 1, fam.nz + 1, net.params(), bf16_mode=True).forward(x)[0]
     err = np.linalg.norm(got[..., :3] - ref[..., :3]) / np.linalg.norm(ref[..., :3])
     assert err <= 2e-2, err
+
+
+def test_combined_forms_quoted_at_the_accuracy_bar(gpu_ctx):
+    """The combined forms (warm start + grouped batch, warm start + mixed
+    precision) are quoted at the loosest tolerance where every member's
+    displacements are within 1e-9 of its FP64 solution converged to 1e-12."""
+    r = forms.form_combined(gpu_ctx, WEIGHTS, members=4, group=2)
+    for key in ("grouped_learned", "mixed_learned"):
+        op = r[key]
+        assert op["meets_bar"] and op["displacement_rel_err"] <= forms.ERR_BAR, (key, op)
+        assert op["max_true_relres"] <= 1.05 * op["rtol"], (key, op)
+        assert op["rtol"] in forms.OP_RTOLS and op["runs"][-1]["rtol"] == op["rtol"]
