@@ -234,6 +234,7 @@ static void ell_release(gf_ell* m) {
   dfree(m->d_n2e_ptr);
   dfree(m->d_n2e);
   dfree(m->d_egrad);
+  dfree(m->d_etens);
   dfree(m->d_val32);
   dfree(m->d_dinv);
   release_solve_plan(m);
@@ -507,9 +508,14 @@ static bool pairs_fits(const gf_mesh* mesh, const gf_ell* m) {
 
 static int assemble_with(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int variant, gf_timing* t) {
   const double h0 = now_ms();
-  if (variant == 3 || variant == 4) {
+  const bool node_rows = variant == 2 || variant == 3 || variant == 4;  // no scatter
+  if (node_rows) {
     int s3 = ensure_n2e(ctx, mesh, m);
     if (s3) return s3;
+  }
+  if (variant == 2 && !m->d_etens) {
+    const int64_t cnt = elem_tensor_doubles(mesh->type, mesh->nelem);
+    GF_HIP_CHECK(ctx, hipMalloc(&m->d_etens, size_t(std::max<int64_t>(cnt, 1)) * sizeof(double)));
   }
   if (variant == 4 && !m->d_egrad) {
     const int64_t cnt = pairs_egrad_doubles(mesh->type, mesh->ngauss, mesh->nelem);
@@ -520,9 +526,9 @@ static int assemble_with(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int varian
   int st = record_start(ctx);
   if (st) return st;
   const size_t slots = size_t(m->n) * size_t(m->width);
-  // the gather and the two-phase kernel write every live slot; the scatters
+  // the gathers and the two-phase kernel write every live slot; the scatters
   // accumulate from zero
-  if (variant != 3 && variant != 4)
+  if (!node_rows)
     GF_HIP_CHECK(ctx, hipMemsetAsync(m->d_val, 0, slots * sizeof(double), ctx->stream));
   AssemblyArgs a{};
   a.type = mesh->type;
@@ -545,18 +551,21 @@ static int assemble_with(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int varian
   // inside the assembly (and its time): the node gather writes it beside
   // the ELL values, the colored scatters are followed by the copy
   const bool bsell = m->d_bval && m->bval_version != 0;
-  a.bval = bsell && (variant == 3 || variant == 4) ? m->d_bval : nullptr;
+  a.bval = bsell && node_rows ? m->d_bval : nullptr;
   // with the copy current the gather writes only the diagonal of the ELL
   // values (what the penalty and the Jacobi preconditioner read); the other
   // slots are materialised from the copy when something reads them
   a.diag = m->d_diag;
-  a.ell_diag_only = bsell && variant == 3;
+  a.ell_diag_only = bsell && (variant == 2 || variant == 3);
   a.bsptr = m->d_bsptr;
   a.binv = m->d_binv;
   a.col = m->d_col;
   a.nelem = mesh->nelem;
-  int64_t launches = variant == 3 || variant == 4 ? 0 : 1;
-  if (variant == 3) {
+  int64_t launches = node_rows ? 0 : 1;
+  if (variant == 2) {
+    GF_HIP_CHECK(ctx, launch_assembly_tensors(ctx->stream, a, m->d_etens));
+    launches += 2;
+  } else if (variant == 3) {
     GF_HIP_CHECK(ctx, launch_assembly_gather(ctx->stream, a));
     ++launches;
   } else if (variant == 4) {
@@ -570,7 +579,7 @@ static int assemble_with(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int varian
       ++launches;
     }
   }
-  if (bsell && variant != 3 && variant != 4) {
+  if (bsell && !node_rows) {
     GF_HIP_CHECK(ctx, launch_bsell_vals(ctx->stream, bsell_view(m), m->dim, m->n, m->d_val,
                                         m->d_rownnz, m->d_bval));
     ++launches;
@@ -608,6 +617,9 @@ int gf_assemble_tune(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int32_t force,
   if (st) return st;
   if (force > 4 || (force == 2 && mesh->dim != 3))
     return set_error(ctx, GF_ERR_ARG, "gf_assemble_tune: variant must be 0, 1, 3, 4 or (3D) 2");
+  if (force == 2 && !gather_fits(mesh, m))
+    return set_error(ctx, GF_ERR_UNSUPPORTED,
+                     "gf_assemble_tune: the matrix-core kernel does not fit this mesh");
   if (force == 3 && !gather_fits(mesh, m))
     return set_error(ctx, GF_ERR_UNSUPPORTED,
                      "gf_assemble_tune: the node-gather kernel does not fit this mesh");
@@ -623,11 +635,21 @@ int gf_assemble_tune(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int32_t force,
       GF_HIP_CHECK(ctx, hipFree(m->d_egrad));
       m->d_egrad = nullptr;
     }
+    if (force != 2 && m->d_etens) {  // nor the matrix-core kernel's tensors
+      GF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+      GF_HIP_CHECK(ctx, hipFree(m->d_etens));
+      m->d_etens = nullptr;
+    }
   } else {
     double best = 1e300;
     int best_v = 0;
-    for (int v : {0, 1, 3}) {
-      if (v == 3 && !gather_fits(mesh, m)) continue;
+    // the matrix-core kernel (2) pays for the 20-node brick only: its element
+    // tensors are 9 x 20 x 20 products over 27 Gauss points, where the
+    // 8-node brick's 9 x 8 x 8 fill a quarter of a 16 x 16 tile and cost
+    // more in tensor traffic than they save
+    for (int v : {0, 1, 3, 2}) {
+      if ((v == 3 || v == 2) && !gather_fits(mesh, m)) continue;
+      if (v == 2 && mesh->type != GF_BRICK20) continue;
       gf_timing t1{}, t2{};
       if ((st = assemble_with(ctx, mesh, m, v, &t1)) || (st = assemble_with(ctx, mesh, m, v, &t2)))
         return st;
@@ -639,6 +661,11 @@ int gf_assemble_tune(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int32_t force,
       }
     }
     m->asm_variant = best_v;
+    if (best_v != 2 && m->d_etens) {
+      GF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+      GF_HIP_CHECK(ctx, hipFree(m->d_etens));
+      m->d_etens = nullptr;
+    }
   }
   if (chosen) *chosen = m->asm_variant;
   return GF_OK;

@@ -108,6 +108,7 @@ struct gf_ell {
   int32_t* d_n2e = nullptr;
   // element gradients at the Gauss points (two-phase assembly; lazily built)
   double* d_egrad = nullptr;
+  double* d_etens = nullptr;  // element tensors of the matrix-core assembly (variant 2)
   std::vector<int32_t> h_elem_color;  // [nelem], 1-based colors
   int64_t nelem = 0;
   int nen = 0;

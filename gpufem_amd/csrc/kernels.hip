@@ -299,45 +299,44 @@ __global__ __launch_bounds__(256) void k_assemble_pm(KArgs a) {
 // pattern, D is isotropic); the result is scattered like k_assemble.
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-template <int NEN>
-__device__ inline double strain_b(const double* ga, int s, int c) {
-  // B_g[s][3a + c] for node a with dN/dx = ga[0..2]; strains xx yy zz xy yz zx
-  switch (s) {
-    case 0: return c == 0 ? ga[0] : 0.0;
-    case 1: return c == 1 ? ga[1] : 0.0;
-    case 2: return c == 2 ? ga[2] : 0.0;
-    case 3: return c == 0 ? ga[1] : (c == 1 ? ga[0] : 0.0);
-    case 4: return c == 1 ? ga[2] : (c == 2 ? ga[1] : 0.0);
-    default: return c == 0 ? ga[2] : (c == 2 ? ga[0] : 0.0);
-  }
-}
+// Element tensors on the matrix cores (phase 1 of variant 2, 3D elements).
+// With s_g = w_g detJ_g E_e and G_g the dN/dx table at Gauss point g, the
+// isotropic element block of nodes a, b is
+//   K_ab[i][j] = c0 T_ii + c2 (T_jj + T_kk)   (i == j)
+//   K_ab[i][j] = c1 T_ij + c2 T_ji            (i != j)
+// with T_pq[a][b] = sum_g s_g G_g[p][a] G_g[q][b]: nine nen x nen products
+// over the Gauss points, which skip the zeros of the strain matrix B.  One
+// wavefront per element forms them as 16x16x4 FP64 MFMA tiles (M = node a,
+// N = node b, K = Gauss point) and writes tens[e][a][p][q][b]; the node
+// gather then reads them instead of recomputing the geometry.
+struct TArgs {
+  KArgs k;
+  double* tens;
+  int64_t nelem;
+};
 
 template <int NEN, int NGP>
-__global__ __launch_bounds__(256) void k_assemble_mfma(KArgs a) {
-  constexpr int NE = 3 * NEN, NT = (NE + 15) / 16, KS = (6 * NGP + 3) / 4;
+__global__ __launch_bounds__(256) void k_elem_tensors(TArgs ta) {
+  constexpr int NT = (NEN + 15) / 16, KS = (NGP + 3) / 4;
   constexpr int EPB = 4;  // one wavefront per element
-  __shared__ double s_dndx[EPB][NGP][3][NEN];
+  __shared__ double s_g[EPB][NGP][3][NEN];
   __shared__ double s_sc[EPB][NGP];
   __shared__ double s_x[EPB][3][NEN];
-  __shared__ int32_t s_node[EPB][NEN];
-  __shared__ int32_t s_elem[EPB];
+  const KArgs& a = ta.k;
   const int tid = threadIdx.x;
   const int64_t base = int64_t(blockIdx.x) * EPB;
   for (int t = tid; t < EPB * NEN; t += 256) {
     const int el = t / NEN, an = t % NEN;
-    if (base + el < a.nce) {
-      const int32_t e = a.elems[base + el];
-      if (an == 0) s_elem[el] = e;
-      const int32_t node = a.conn[int64_t(e) * NEN + an] - 1;
-      s_node[el][an] = node;
+    if (base + el < ta.nelem) {
+      const int64_t node = a.conn[(base + el) * NEN + an] - 1;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) s_x[el][d][an] = a.coords[int64_t(node) * 3 + d];
+      for (int d = 0; d < 3; ++d) s_x[el][d][an] = a.coords[node * 3 + d];
     }
   }
   __syncthreads();
   for (int t = tid; t < EPB * NGP; t += 256) {
     const int el = t / NGP, g = t % NGP;
-    if (base + el >= a.nce) continue;
+    if (base + el >= ta.nelem) continue;
     const double* dN = c_tab + a.tab + g * 3 * NEN;
     double J[3][3];
 #pragma unroll
@@ -359,67 +358,63 @@ __global__ __launch_bounds__(256) void k_assemble_mfma(KArgs a) {
         double acc = 0.0;
 #pragma unroll
         for (int k = 0; k < 3; ++k) acc += Ji[i][k] * dN[k * NEN + n];
-        s_dndx[el][g][i][n] = acc;
+        s_g[el][g][i][n] = acc;
       }
     const double w = c_tab[a.tab + NGP * 3 * NEN + g];
-    const double Ee = a.elemE ? a.elemE[s_elem[el]] : a.E;
+    const double Ee = a.elemE ? a.elemE[base + el] : a.E;
     s_sc[el][g] = w * det * Ee;
   }
   __syncthreads();
   const int el = tid >> 6, lane = tid & 63;
-  if (base + el >= a.nce) return;  // the whole wavefront leaves; no barrier follows
+  const int64_t e = base + el;
+  if (e >= ta.nelem) return;  // the whole wavefront leaves; no barrier follows
   const int li = lane & 15, lk = lane >> 4;
-  f64x4 acc[NT][NT];
+  double* te = ta.tens + e * (9 * NEN * NEN);
+  for (int p = 0; p < 3; ++p) {
+    f64x4 acc[3][NT][NT];
 #pragma unroll
-  for (int i = 0; i < NT; ++i)
+    for (int q = 0; q < 3; ++q)
 #pragma unroll
-    for (int j = 0; j < NT; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
-  for (int ks = 0; ks < KS; ++ks) {
-    const int k = ks * 4 + lk;
-    const int g = k / 6, s = k - 6 * (k / 6);
-    const bool kv = g < NGP;
-    const double sc = kv ? s_sc[el][g] : 0.0;
-    double av[NT], bv[NT];
+      for (int i = 0; i < NT; ++i)
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const int I = t * 16 + li;
-      double x = 0.0, y = 0.0;
-      if (kv && I < NE) {
-        const int nd = I / 3, c = I - 3 * nd;
-        double ga[3];
+        for (int j = 0; j < NT; ++j) acc[q][i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int ks = 0; ks < KS; ++ks) {
+      // lane (li, lk): A[row li][k lk] and B[k lk][column li] of the step
+      const int g = ks * 4 + lk;
+      const bool gv = g < NGP;
+      const double sc = gv ? s_sc[el][g] : 0.0;
+      double av[NT], bv[3][NT];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) ga[d] = s_dndx[el][g][d][nd];
-        x = strain_b<NEN>(ga, s, c);
-        // (D B_g)[s][I]: isotropic D (normal c0 / c1, shear c2)
-        y = s < 3 ? (s == c ? a.c0 : a.c1) * ga[c] : a.c2 * x;
-        y *= sc;
+      for (int t = 0; t < NT; ++t) {
+        const int n = t * 16 + li;
+        const bool v = gv && n < NEN;
+        av[t] = v ? sc * s_g[el][g][p][n] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) bv[q][t] = v ? s_g[el][g][q][n] : 0.0;
       }
-      av[t] = x;
-      bv[t] = y;
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+          for (int tj = 0; tj < NT; ++tj)
+            acc[q][ti][tj] =
+                __builtin_amdgcn_mfma_f64_16x16x4f64(av[ti], bv[q][tj], acc[q][ti][tj], 0, 0, 0);
     }
 #pragma unroll
-    for (int ti = 0; ti < NT; ++ti)
+    for (int q = 0; q < 3; ++q)
 #pragma unroll
-      for (int tj = 0; tj < NT; ++tj)
-        acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ti], bv[tj], acc[ti][tj], 0, 0, 0);
+      for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < NT; ++tj)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            // v_mfma_f64_16x16x4f64 result: lane l, register r holds
+            // D[l/16 + 4r][l%16] (lane layout measured on gfx950)
+            const int an = ti * 16 + lk + 4 * r, bn = tj * 16 + li;
+            if (an < NEN && bn < NEN) te[(an * 9 + p * 3 + q) * NEN + bn] = acc[q][ti][tj][r];
+          }
   }
-  const int32_t e = s_elem[el];
-#pragma unroll
-  for (int ti = 0; ti < NT; ++ti)
-#pragma unroll
-    for (int tj = 0; tj < NT; ++tj)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        // v_mfma_f64_16x16x4f64 result: lane l, register r holds
-        // D[l/16 + 4r][l%16] (lane layout measured on gfx950)
-        const int I = ti * 16 + lk + 4 * r, J = tj * 16 + li;
-        if (I < NE && J < NE) {
-          const int an = I / 3, ci = I - 3 * an, bn = J / 3, cj = J - 3 * bn;
-          const int64_t slot = a.slot[(int64_t(e) * NEN + an) * NEN + bn];
-          const int64_t row = int64_t(s_node[el][an]) * 3 + ci;
-          a.val[row + (slot * 3 + cj) * a.n] += acc[ti][tj][r];
-        }
-      }
 }
 
 // ---------------------------------------------------------------------------
@@ -446,6 +441,8 @@ struct GArgs {
   // ell_diag_only: of the ELL values only the diagonal (slot diag[row])
   const int32_t* diag;
   int ell_diag_only;
+  // element tensors (variant 2): tens[e][a][p][q][b]
+  const double* tens;
 };
 
 // A unit is RPT rows of one node (UPN units per node).  LPU lanes share a
@@ -462,9 +459,11 @@ struct GArgs {
 // the accumulator registers.  With RPT < DIM (one row per unit) the row's
 // entries are accumulated directly and BCH = NEN.  GPU: unroll factor of the
 // Gauss-point loop.
-template <int DIM, int NEN, int NGP, int MAXS, int RPT, int LPU, int UPW, int BCH, int GPU = 1>
+template <int DIM, int NEN, int NGP, int MAXS, int RPT, int LPU, int UPW, int BCH, int GPU = 1,
+          int SRC = 0>
 __global__ __launch_bounds__(64) void k_assemble_gather(GArgs ga) {
   static_assert(LPU * UPW <= 64, "a workgroup is one wavefront");
+  static_assert(SRC == 0 || (RPT == DIM && DIM == 3), "element tensors: 3D node units");
   static_assert(NEN % BCH == 0 && (RPT == DIM || BCH == NEN), "neighbour chunks");
   constexpr int UPN = DIM / RPT;  // units per node
   constexpr int ACC = MAXS * RPT * DIM;
@@ -496,13 +495,16 @@ __global__ __launch_bounds__(64) void k_assemble_gather(GArgs ga) {
       const int64_t e = packed >> 5;
       const int la = packed & 31;
       double x[NEN][DIM];
+      double Ee = 0.0;
+      if constexpr (SRC == 0) {
 #pragma unroll
-      for (int n = 0; n < NEN; ++n) {
-        const int64_t node = a.conn[e * NEN + n] - 1;
+        for (int n = 0; n < NEN; ++n) {
+          const int64_t node = a.conn[e * NEN + n] - 1;
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) x[n][d] = a.coords[node * DIM + d];
+          for (int d = 0; d < DIM; ++d) x[n][d] = a.coords[node * DIM + d];
+        }
+        Ee = a.elemE ? a.elemE[e] : a.E;
       }
-      const double Ee = a.elemE ? a.elemE[e] : a.E;
       const uint8_t* sl = a.slot + (e * NEN + la) * NEN;
       for (int c0 = 0; c0 < NEN; c0 += BCH) {
         double kl[BCH][RPT][DIM];
@@ -512,8 +514,18 @@ __global__ __launch_bounds__(64) void k_assemble_gather(GArgs ga) {
           for (int r = 0; r < RPT; ++r)
 #pragma unroll
             for (int j = 0; j < DIM; ++j) kl[b][r][j] = 0.0;
+        if constexpr (SRC == 1) {
+          // the element tensors T_pq[a][b] of node a = la, b in the chunk
+          const double* tt = ga.tens + (e * NEN + la) * (9 * NEN) + c0;
+#pragma unroll
+          for (int b = 0; b < BCH; ++b)
+#pragma unroll
+            for (int p = 0; p < 3; ++p)
+#pragma unroll
+              for (int q = 0; q < 3; ++q) kl[b][p][q] = tt[(p * 3 + q) * NEN + b];
+        }
 #pragma unroll(GPU)
-        for (int g = 0; g < NGP; ++g) {
+        for (int g = 0; g < (SRC == 0 ? NGP : 0); ++g) {
           const double* dN = c_tab + a.tab + g * DIM * NEN;
           double J[DIM][DIM];
 #pragma unroll
@@ -884,16 +896,6 @@ constexpr int pm_epb() {
 
 template <int DIM, int NEN, int NGP>
 hipError_t launch_asm(hipStream_t s, const KArgs& k, int variant) {
-  if (variant == 2) {
-    if constexpr (DIM == 3) {
-      const int64_t blocks = (k.nce + 3) / 4;
-      if (blocks == 0) return hipSuccess;
-      hipLaunchKernelGGL((k_assemble_mfma<NEN, NGP>), dim3(unsigned(blocks)), dim3(256), 0, s, k);
-      return hipGetLastError();
-    } else {
-      return hipErrorInvalidValue;
-    }
-  }
   if (variant == 1) {
     constexpr int EPB = pm_epb<DIM, NEN, NGP>();
     const int64_t blocks = (k.nce + EPB - 1) / EPB;
@@ -1130,11 +1132,14 @@ __global__ __launch_bounds__(256) void k_assemble_pairs(PArgs pa) {
 }
 
 namespace {
-template <int DIM, int NEN, int NGP, int MAXS, int RPT, int LPU, int UPW, int BCH, int GPU = 1>
+template <int DIM, int NEN, int NGP, int MAXS, int RPT, int LPU, int UPW, int BCH, int GPU = 1,
+          int SRC = 0>
 hipError_t launch_gather_cfg(hipStream_t s, const GArgs& g) {
   const int64_t units = g.nnodes * (DIM / RPT);
   if (units == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_assemble_gather<DIM, NEN, NGP, MAXS, RPT, LPU, UPW, BCH, GPU>),
+  // one tile of UPW units per workgroup: a wavefront ends after its stores
+  // instead of waiting for them before the loads of a next tile
+  hipLaunchKernelGGL((k_assemble_gather<DIM, NEN, NGP, MAXS, RPT, LPU, UPW, BCH, GPU, SRC>),
                      dim3(grid_for(units, UPW, 1 << 30)), dim3(64), 0, s, g);
   return hipGetLastError();
 }
@@ -1244,11 +1249,9 @@ int gather_max_slots(int type) {
   }
 }
 
-hipError_t launch_assembly_gather(hipStream_t s, const AssemblyArgs& a) {
-  ElemInfo ei;
-  if (!elem_info(a.type, &ei) || (a.ngauss != 2 && a.ngauss != 3) || gather_max_slots(a.type) == 0)
-    return hipErrorInvalidValue;
-  GArgs g;
+namespace {
+GArgs gather_args(const AssemblyArgs& a) {
+  GArgs g{};
   KArgs& k = g.k;
   k.conn = a.conn;
   k.coords = a.coords;
@@ -1273,6 +1276,50 @@ hipError_t launch_assembly_gather(hipStream_t s, const AssemblyArgs& a) {
   g.binv = a.binv;
   g.diag = a.diag;
   g.ell_diag_only = a.bval && a.ell_diag_only ? 1 : 0;
+  g.tens = nullptr;
+  return g;
+}
+
+template <int NEN, int NGP, int MAXS, int LPU, int UPW, int BCH>
+hipError_t launch_tensors(hipStream_t s, const AssemblyArgs& a, double* tens) {
+  GArgs g = gather_args(a);
+  g.tens = tens;
+  if (a.nelem > 0) {
+    TArgs ta{g.k, tens, a.nelem};
+    hipLaunchKernelGGL((k_elem_tensors<NEN, NGP>), dim3(unsigned((a.nelem + 3) / 4)), dim3(256),
+                       0, s, ta);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return launch_gather_cfg<3, NEN, NGP, MAXS, 3, LPU, UPW, BCH, 1, 1>(s, g);
+}
+}  // namespace
+
+int64_t elem_tensor_doubles(int type, int64_t nelem) {
+  ElemInfo ei;
+  if (!elem_info(type, &ei) || ei.dim != 3) return 0;
+  return nelem * 9 * int64_t(ei.nen) * ei.nen;
+}
+
+hipError_t launch_assembly_tensors(hipStream_t s, const AssemblyArgs& a, double* tens) {
+  if (a.ngauss != 2 && a.ngauss != 3) return hipErrorInvalidValue;
+  const bool g3 = a.ngauss == 3;
+  switch (a.type) {
+    case GF_BRICK8:
+      return g3 ? launch_tensors<8, 27, 27, 4, 16, 8>(s, a, tens)
+                : launch_tensors<8, 8, 27, 4, 16, 8>(s, a, tens);
+    case GF_BRICK20:
+      return g3 ? launch_tensors<20, 27, 81, 8, 8, 5>(s, a, tens)
+                : launch_tensors<20, 8, 81, 8, 8, 5>(s, a, tens);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_assembly_gather(hipStream_t s, const AssemblyArgs& a) {
+  ElemInfo ei;
+  if (!elem_info(a.type, &ei) || (a.ngauss != 2 && a.ngauss != 3) || gather_max_slots(a.type) == 0)
+    return hipErrorInvalidValue;
+  const GArgs g = gather_args(a);
   const bool g3 = a.ngauss == 3;
   switch (a.type) {
     case GF_Q4: return g3 ? launch_gather2<4, 9, 9>(s, g) : launch_gather2<4, 4, 9>(s, g);

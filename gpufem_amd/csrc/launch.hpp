@@ -64,6 +64,10 @@ hipError_t launch_assembly_pairs(hipStream_t s, const AssemblyArgs& a, double* e
 // every element around the node and adds them in LDS, then writes its rows
 // of the ELL values once (every live slot).  Applicable when the node has at
 // most gather_max_slots(type) neighbours (0: not available for the type).
+// Variant 2 (3D): element tensors on the matrix cores into tens (a buffer of
+// elem_tensor_doubles(type, nelem) doubles), then the node gather over them.
+int64_t elem_tensor_doubles(int type, int64_t nelem);
+hipError_t launch_assembly_tensors(hipStream_t s, const AssemblyArgs& a, double* tens);
 int gather_max_slots(int type);
 hipError_t launch_assembly_gather(hipStream_t s, const AssemblyArgs& a);
 

@@ -74,7 +74,8 @@ int run_multi(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, const double* B
   B.col = is_bsell_variant(P->variant) ? o.col : m->d_col;
   B.bsell_d = is_bsell_variant(P->variant) ? m->dim : 0;
   B.bs = o.bs;
-  B.spmm_lds = multi_spmm_lds(ctx->lds_per_cu);
+  // measured on 3x3 blocks (Brick20 48^3, 32 load cases); 2D keeps the default
+  B.spmm_lds = B.bsell_d == 3 ? multi_spmm_lds(ctx->lds_per_cu) : 0;
   B.rownnz = m->d_rownnz;
   B.dinv = opts.precond == GF_PRECOND_JACOBI ? m->d_dinv : nullptr;
   B.b = W->b;

@@ -12,20 +12,22 @@ from gpufem_amd import problems as P
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("t,dims,bsell", [(gf.GF_Q8, (20, 10), 9), (gf.GF_Q4, (33, 17), 7),
-                                          (gf.GF_BRICK8, (7, 6, 5), 10),
-                                          (gf.GF_BRICK8, (5, 4, 3), 8),
-                                          (gf.GF_BRICK20, (4, 3, 3), 10)])
-def test_gather_writes_block_sliced_copy(gpu_ctx, t, dims, bsell):
-    """With the copy built, the node gather writes it beside the ELL values
-    and the penalty updates its diagonal: its products stay bitwise equal to
-    the thread-per-row ELL kernel's, with no re-layout in between."""
+@pytest.mark.parametrize("t,dims,bsell,asm", [(gf.GF_Q8, (20, 10), 9, 3), (gf.GF_Q4, (33, 17), 7, 3),
+                                              (gf.GF_BRICK8, (7, 6, 5), 10, 3),
+                                              (gf.GF_BRICK8, (5, 4, 3), 8, 3),
+                                              (gf.GF_BRICK20, (4, 3, 3), 10, 3),
+                                              (gf.GF_BRICK20, (4, 3, 3), 10, 2)])
+def test_gather_writes_block_sliced_copy(gpu_ctx, t, dims, bsell, asm):
+    """With the copy built, the node gather (asm 3; asm 2 over the element
+    tensors of the matrix cores) writes it and the penalty updates its
+    diagonal: its products stay bitwise equal to the thread-per-row ELL
+    kernel's, with no re-layout in between."""
     p = P.structured(t, *dims)
     mesh = gpu_ctx.upload_mesh(p.elem_type, p.ngauss, p.conn, p.coords, p.E, p.nu)
     K = gpu_ctx.symbolic(mesh)
     gpu_ctx.assemble(mesh, K)
     gpu_ctx.tune(K, bsell)                       # builds the copy from the ELL values
-    gpu_ctx.assemble_tune(mesh, K, 3)            # node gather from here on
+    gpu_ctx.assemble_tune(mesh, K, asm)          # node gather from here on
     x = np.random.default_rng(4).normal(size=p.ndof)
     rng = np.random.default_rng(7)
     for step in range(2):
@@ -47,9 +49,11 @@ def test_gather_writes_block_sliced_copy(gpu_ctx, t, dims, bsell):
     assert np.abs(y0 - yo).max() <= 1e-12 * np.abs(yo).max()
 
 
-@pytest.mark.parametrize("t,dims,bsell", [(gf.GF_Q8, (20, 10), 9), (gf.GF_BRICK8, (7, 6, 5), 10),
-                                          (gf.GF_BRICK20, (4, 3, 3), 10)])
-def test_stale_ell_values_materialise(gpu_ctx, t, dims, bsell):
+@pytest.mark.parametrize("t,dims,bsell,asm", [(gf.GF_Q8, (20, 10), 9, 3),
+                                              (gf.GF_BRICK8, (7, 6, 5), 10, 3),
+                                              (gf.GF_BRICK20, (4, 3, 3), 10, 3),
+                                              (gf.GF_BRICK20, (4, 3, 3), 10, 2)])
+def test_stale_ell_values_materialise(gpu_ctx, t, dims, bsell, asm):
     """Once the copy exists the node gather writes it and only the diagonal
     of the ELL values; every reader of the other slots (download, the
     elimination, the ELL SpMV kernels) gets them materialised from the copy:
@@ -62,7 +66,7 @@ def test_stale_ell_values_materialise(gpu_ctx, t, dims, bsell):
     K2 = gpu_ctx.symbolic(mesh)     # without a copy: every ELL value written
     K3 = gpu_ctx.symbolic(mesh)     # copy built from fully written ELL values
     for M in (K, K2, K3):
-        gpu_ctx.assemble_tune(mesh, M, 3)
+        gpu_ctx.assemble_tune(mesh, M, asm)
     gpu_ctx.tune(K, bsell)
     gpu_ctx.tune(K2, 0)
     for M in (K, K2, K3):
