@@ -307,8 +307,9 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 // with T_pq[a][b] = sum_g s_g G_g[p][a] G_g[q][b]: nine nen x nen products
 // over the Gauss points, which skip the zeros of the strain matrix B.  One
 // wavefront per element forms them as 16x16x4 FP64 MFMA tiles (M = node a,
-// N = node b, K = Gauss point) and writes tens[e][a][p][q][b]; the node
-// gather then reads them instead of recomputing the geometry.
+// N = node b, K = Gauss point) and writes tens[e][a][p][b][q] (the gather
+// reads a chunk of neighbours b as three contiguous runs); the node gather
+// then reads them instead of recomputing the geometry.
 struct TArgs {
   KArgs k;
   double* tens;
@@ -344,14 +345,14 @@ __global__ __launch_bounds__(256) void k_elem_tensors(TArgs ta) {
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         double acc = 0.0;
-#pragma unroll
+#pragma unroll(NEN > 8 ? 4 : NEN)
         for (int n = 0; n < NEN; ++n) acc += dN[i * NEN + n] * s_x[el][j][n];
         J[i][j] = acc;
       }
     double Ji[3][3];
     const double det = det_inv<3>(J, Ji);
     if (!(det > 0.0)) *a.bad = 1;
-#pragma unroll
+#pragma unroll(NEN > 8 ? 4 : NEN)
     for (int n = 0; n < NEN; ++n)
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
@@ -412,7 +413,7 @@ __global__ __launch_bounds__(256) void k_elem_tensors(TArgs ta) {
             // v_mfma_f64_16x16x4f64 result: lane l, register r holds
             // D[l/16 + 4r][l%16] (lane layout measured on gfx950)
             const int an = ti * 16 + lk + 4 * r, bn = tj * 16 + li;
-            if (an < NEN && bn < NEN) te[(an * 9 + p * 3 + q) * NEN + bn] = acc[q][ti][tj][r];
+            if (an < NEN && bn < NEN) te[((an * 3 + p) * NEN + bn) * 3 + q] = acc[q][ti][tj][r];
           }
   }
 }
@@ -441,7 +442,7 @@ struct GArgs {
   // ell_diag_only: of the ELL values only the diagonal (slot diag[row])
   const int32_t* diag;
   int ell_diag_only;
-  // element tensors (variant 2): tens[e][a][p][q][b]
+  // element tensors (variant 2): tens[e][a][p][b][q]
   const double* tens;
 };
 
@@ -516,13 +517,13 @@ __global__ __launch_bounds__(64) void k_assemble_gather(GArgs ga) {
             for (int j = 0; j < DIM; ++j) kl[b][r][j] = 0.0;
         if constexpr (SRC == 1) {
           // the element tensors T_pq[a][b] of node a = la, b in the chunk
-          const double* tt = ga.tens + (e * NEN + la) * (9 * NEN) + c0;
+          const double* tt = ga.tens + (e * NEN + la) * (9 * NEN) + c0 * 3;
 #pragma unroll
-          for (int b = 0; b < BCH; ++b)
+          for (int p = 0; p < 3; ++p)
 #pragma unroll
-            for (int p = 0; p < 3; ++p)
+            for (int b = 0; b < BCH; ++b)
 #pragma unroll
-              for (int q = 0; q < 3; ++q) kl[b][p][q] = tt[(p * 3 + q) * NEN + b];
+              for (int q = 0; q < 3; ++q) kl[b][p][q] = tt[(p * NEN + b) * 3 + q];
         }
 #pragma unroll(GPU)
         for (int g = 0; g < (SRC == 0 ? NGP : 0); ++g) {
@@ -1309,8 +1310,10 @@ hipError_t launch_assembly_tensors(hipStream_t s, const AssemblyArgs& a, double*
       return g3 ? launch_tensors<8, 27, 27, 4, 16, 8>(s, a, tens)
                 : launch_tensors<8, 8, 27, 4, 16, 8>(s, a, tens);
     case GF_BRICK20:
-      return g3 ? launch_tensors<20, 27, 81, 8, 8, 5>(s, a, tens)
-                : launch_tensors<20, 8, 81, 8, 8, 5>(s, a, tens);
+      // gather phase: 8 lanes x 8 nodes, neighbours 10 at a time (48^3: 4.19 ms;
+      // chunks of 4/5: 4.22-4.23, 16 lanes x 4 nodes: 4.40-4.49, 4 x 8: 4.46)
+      return g3 ? launch_tensors<20, 27, 81, 8, 8, 10>(s, a, tens)
+                : launch_tensors<20, 8, 81, 8, 8, 10>(s, a, tens);
     default: return hipErrorInvalidValue;
   }
 }
