@@ -650,6 +650,12 @@ int gf_assemble_tune(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int32_t force,
     for (int v : {0, 1, 3, 2}) {
       if ((v == 3 || v == 2) && !gather_fits(mesh, m)) continue;
       if (v == 2 && mesh->type != GF_BRICK20) continue;
+      if (v == 2 && !m->d_etens) {
+        // the tensors must leave most of the device free: else not tried
+        size_t free_b = 0, total_b = 0;
+        const double need = double(elem_tensor_doubles(mesh->type, mesh->nelem)) * sizeof(double);
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || need > 0.5 * double(free_b)) continue;
+      }
       gf_timing t1{}, t2{};
       if ((st = assemble_with(ctx, mesh, m, v, &t1)) || (st = assemble_with(ctx, mesh, m, v, &t2)))
         return st;
