@@ -36,7 +36,8 @@ from . import problems as P
 RTOL = 1e-8
 REF_RTOL = 1e-12   # the converged FP64 reference
 ERR_BAR = 1e-9     # displacement accuracy required of every form
-OP_RTOLS = (1e-8, 1e-9, 1e-10, 1e-11)
+# operating-point search: half-decade steps from the reference's 1e-8
+OP_RTOLS = (1e-8, 3e-9, 1e-9, 3e-10, 1e-10, 3e-11, 1e-11, 3e-12)
 
 
 def _log(msg):
