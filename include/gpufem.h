@@ -172,30 +172,35 @@ int gf_ell_download(gf_ctx* ctx, const gf_ell* m, double* val, int32_t* col,
                     int32_t* rownnz);
 
 /* Numeric assembly: Ke by Gauss quadrature fused with the colored,
- * atomic-free scatter (one launch per color), or with the node gather
- * (variant 3, one launch); every stored value is recomputed. */
+ * atomic-free scatter (one launch per color), or with a node gather
+ * (variants 2 and 3); every stored value is recomputed.  Replaces
+ * StiffAlgo::CalcGlobalStiffness (src/FEM/StiffAlgo.h:46). */
 int gf_assemble(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, gf_timing* t);
 /* Assembly kernel of this matrix: 0 = one thread per (element, node pair)
  * with the element outermost; 1 = pair-major, consecutive threads on the same
  * node pair of consecutive elements of a color (neighbouring rows in the
- * scatter); 2 (Brick8/Brick20 only, forced only) = one wavefront per element
- * forming Ke = sum_g B^T (D B) w detJ E as one GEMM on the FP64 matrix
- * cores; 3 (Q4, Q8, Brick8 with at most 9 / 21 / 27 neighbours per node) =
- * node gather, one thread per node row(s) forming the node's rows of every
- * surrounding element's Ke and writing its ELL rows once (no coloring, no
- * read-modify-write); 4 (forced only; at most 9 / 21 / 27 / 81 neighbours per
- * node) = two phases, the gradients of every element at its Gauss points
- * computed once into a buffer the matrix keeps while 4 is its variant
- * (nelem x ngp x (nen dim + 1) doubles), then one thread per (node,
- * neighbour node) summing that dim x dim block over the shared elements and
- * writing it once.  0 and 1 add the same terms in the same order
- * (bitwise-identical values); 2, 3 and 4 sum in other orders (equal to
- * rounding).  force_variant 0..4 fixes it (GF_ERR_UNSUPPORTED where 3 or 4
- * does not fit); -1 assembles twice with each of 0, 1 and 3 (where it fits)
- * and keeps the fastest (gf_assemble does this on its first call for meshes
- * of 65536 elements or more).  When
- * the solver's block-sliced copy of the matrix exists, 3 and 4 write it
- * beside the ELL values and the others are followed by the copy, inside the
+ * scatter); 2 (Brick8/Brick20 with at most 27 / 81 neighbours per node) =
+ * element tensors T_pq[a][b] = sum_g w detJ E dNa/dx_p dNb/dx_q formed on
+ * the FP64 matrix cores (one wavefront per element, into a buffer of
+ * nelem x 9 x nen^2 doubles the matrix keeps while 2 is its variant), then
+ * the node gather over them; 3 (at most 9 / 21 / 27 / 81 neighbours per
+ * node for Q4 / Q8 / Brick8 / Brick20) = node gather, the lanes of a node
+ * (or node row) forming its rows of every surrounding element's Ke and
+ * writing them once (no coloring, no read-modify-write); 4 (forced only) =
+ * two phases, the gradients of every element at its Gauss points computed
+ * once into a buffer the matrix keeps while 4 is its variant (nelem x ngp x
+ * (nen dim + 1) doubles), then one thread per (node, neighbour node)
+ * summing that dim x dim block over the shared elements and writing it
+ * once.  0 and 1 add the same terms in the same order (bitwise-identical
+ * values); 2, 3 and 4 sum in other orders (equal to rounding).
+ * force_variant 0..4 fixes it (GF_ERR_UNSUPPORTED where 2, 3 or 4 does not
+ * fit); -1 assembles twice with each of 0, 1, 3 and, for Brick20 when its
+ * tensors take at most half the free device memory, 2, and keeps the
+ * fastest (gf_assemble does this on its first call for meshes of 65536
+ * elements or more).  When the solver's block-sliced copy of the matrix
+ * exists, 2, 3 and 4 write it (2 and 3 with only the diagonal of the ELL
+ * values; the other ELL values are rebuilt from the copy by the first call
+ * that reads them) and the others are followed by the copy, inside the
  * timing.  ms_per_variant (may be NULL): GF_ASM_VARIANTS entries, the better
  * of two timings of each variant timed, -1 otherwise. */
 #define GF_ASM_VARIANTS 5
