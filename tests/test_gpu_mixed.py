@@ -84,16 +84,17 @@ def test_mixed_ill_conditioned_continues_in_fp64(gpu_ctx):
     """A slender Q8 cantilever (128 x 8) whose Jacobi-scaled condition number
     is 1.5e8 (u32 kappa about 9; CPU emulation: FP32 iterations, even with
     FP64 vectors, stop converging on such plates).  The condition estimate
-    from the CG coefficients crosses 1/u32 and the solve continues in FP64
-    from the current x: the answer still meets the tolerance and the
-    oracle's displacements."""
+    from the CG coefficients crosses 0.05/u32 and the solve continues in
+    FP64 (from the FP32 iterate or from zero, whichever has the lower energy
+    functional): the answer still meets the tolerance and the oracle's
+    displacements, and the FP32 part costs at most a few check intervals."""
     p = P.config_q8_plate(128, 8)
     mesh, K = setup(gpu_ctx, p)
     x64, s64 = gpu_ctx.pcg(K, p.f)
     assert s64.converged and s64.true_relres <= 1.05e-8
     xm, sm = gpu_ctx.pcg(K, p.f, precision=gf.GF_PREC_MIXED)
     assert sm.converged and sm.true_relres <= 1.05e-8 and sm.fallbacks == 1
-    assert sm.iterations <= 3 * s64.iterations + 200, (sm.iterations, s64.iterations)
+    assert sm.iterations <= s64.iterations + 6 * 256, (sm.iterations, s64.iterations)
     xo = oracle_solution(p)
     assert np.linalg.norm(xm - xo) <= forms.ERR_BAR * np.linalg.norm(xo)
     assert np.linalg.norm(x64 - xo) <= forms.ERR_BAR * np.linalg.norm(xo)
