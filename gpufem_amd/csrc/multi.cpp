@@ -9,7 +9,7 @@
 struct gf_multi_ws {
   int64_t n = 0;
   int S = 0;
-  double *b = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  double *b = nullptr, *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;
   double *part1 = nullptr, *part2 = nullptr, *res = nullptr;
   void* st = nullptr;
 };
@@ -26,7 +26,7 @@ constexpr int kMaxRestarts = 4;  // true-residual restarts
 void release_multi(gf_solve_plan* P) {
   gf_multi_ws* W = P->multi;
   if (!W) return;
-  double* v[] = {W->b, W->x, W->r, W->z, W->p, W->q, W->part1, W->part2, W->res};
+  double* v[] = {W->b, W->x, W->r, W->p, W->q, W->part1, W->part2, W->res};
   for (double* d : v)
     if (d) (void)hipFree(d);
   if (W->st) (void)hipFree(W->st);
@@ -49,7 +49,7 @@ int run_multi(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, const double* B
     const size_t vb = size_t(n) * size_t(S) * sizeof(double);
     const int nb = multi_blocks(n, S);
     hipError_t e = hipSuccess;
-    double** v[] = {&W->b, &W->x, &W->r, &W->z, &W->p, &W->q};
+    double** v[] = {&W->b, &W->x, &W->r, &W->p, &W->q};
     for (double** d : v)
       if (e == hipSuccess) e = hipMalloc(d, vb);
     if (e == hipSuccess) e = hipMalloc(&W->part1, size_t(S) * nb * sizeof(double));
@@ -81,7 +81,6 @@ int run_multi(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, const double* B
   B.b = W->b;
   B.x = W->x;
   B.r = W->r;
-  B.z = W->z;
   B.p = W->p;
   B.q = W->q;
   B.part1 = W->part1;
@@ -98,6 +97,7 @@ int run_multi(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, const double* B
       for (int it = 0; it < kChunk; ++it) GF_HIP_CHECK(ctx, launch_multi_iteration(ctx->stream, B));
       GF_HIP_CHECK(ctx, read_multi_state(ctx->stream, B, h));
     }
+    GF_HIP_CHECK(ctx, launch_multi_x_finish(ctx->stream, B));
     for (int k = 0; k < ncols; ++k) {
       col_total[size_t(k)] += h->col_it[k];
       h->col_it[k] = col_total[size_t(k)];

@@ -49,7 +49,6 @@ __global__ __launch_bounds__(kNT) void k_b_init(MultiBuffers B) {
     if (!B.use_x0) B.x[o] = 0.0;
     const double zi = B.dinv ? B.dinv[i] * ri : ri;
     B.r[o] = ri;
-    B.z[o] = zi;
     B.p[o] = zi;
     rz += ri * zi;
     rr += ri * ri;
@@ -98,37 +97,40 @@ __global__ __launch_bounds__(kNT) void k_b_init_finish(MultiBuffers B, int nb, i
   }
 }
 
-// q = K p and the per-column partials of p.q
+// q = K p and the per-column partials of p.q.  Launched at two workgroups per
+// CU (padded LDS), i.e. two waves per SIMD, so the compiler may spend up to
+// 256 registers keeping every load of a step in flight.
 template <int S, int D, int NB = 2>
-__global__ __launch_bounds__(kNT) void k_b_spmm_pq_bs(MultiBuffers B) {
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_b_spmm_pq_bs(
+    MultiBuffers B) {
   const MultiState<S>* st = static_cast<const MultiState<S>*>(B.st);
   if (st->done) return;
-  __shared__ double sh[kNT * kC];
-  double pq[kC] = {};
-  for_chunk_node_rows<S, D, double, double, false, NB>(spmm_args(B), B.val, B.p,
-                                                       [&](int32_t R, int c0,
-                                                           const double (*acc)[kC]) {
+  __shared__ double sh[kNT * kCB];
+  double pq[kCB] = {};
+  for_chunk_node_rows<S, D, double, double, false, NB, true, kCB>(
+      spmm_args(B), B.val, B.p, [&](int32_t R, int c0, const double (*acc)[kCB]) {
 #pragma unroll
-    for (int r = 0; r < D; ++r) {
-      const int64_t o = (int64_t(D) * R + r) * S + c0;
-      double pv[kC];
-      load_chunk(B.p + o, pv);
-      store_chunk(B.q + o, acc[r]);
+        for (int r = 0; r < D; ++r) {
+          const int64_t o = (int64_t(D) * R + r) * S;
+          double pv[kCB];
+          load_split<S, kCB>(B.p + o, c0, pv);
+          store_split<S, kCB>(B.q + o, c0, acc[r]);
 #pragma unroll
-      for (int j = 0; j < kC; ++j) pq[j] += pv[j] * acc[r][j];
-    }
-  });
-  chunk_partials<S>(pq, sh, B.part1);
+          for (int j = 0; j < kCB; ++j) pq[j] += pv[j] * acc[r][j];
+        }
+      });
+  chunk_partials<S, true, kCB>(pq, sh, B.part1);
 }
 
 template <int S, int D>
 __global__ __launch_bounds__(kNT) void k_b_spmm_bs(MultiBuffers B, const double* __restrict__ xin,
                                                    double* __restrict__ yout) {
-  for_chunk_node_rows<S, D>(spmm_args(B), B.val, xin, [&](int32_t R, int c0,
-                                                         const double (*acc)[kC]) {
+  for_chunk_node_rows<S, D, double, double, false, 2, true, kCB>(
+      spmm_args(B), B.val, xin, [&](int32_t R, int c0, const double (*acc)[kCB]) {
 #pragma unroll
-    for (int r = 0; r < D; ++r) store_chunk(yout + (int64_t(D) * R + r) * S + c0, acc[r]);
-  });
+        for (int r = 0; r < D; ++r)
+          store_split<S, kCB>(yout + (int64_t(D) * R + r) * S, c0, acc[r]);
+      });
 }
 
 template <int S>
@@ -183,14 +185,12 @@ __global__ __launch_bounds__(kNT) void k_b_update(MultiBuffers B) {
   __syncthreads();
   const int nb = gridDim.x;
   double rz = 0, rr = 0;
+  // x and p are updated together in k_b_direction; z = D^-1 r is not stored
   for_group_rows<S>(B.n, [&](int64_t i, int k) {
     const int64_t o = i * S + k;
-    const double a = s_alpha[k];
-    B.x[o] += a * B.p[o];
-    const double ri = B.r[o] - a * B.q[o];
+    const double ri = B.r[o] - s_alpha[k] * B.q[o];
     const double zi = B.dinv ? B.dinv[i] * ri : ri;
     B.r[o] = ri;
-    B.z[o] = zi;
     rz += ri * zi;
     rr += ri * ri;
   });
@@ -236,16 +236,40 @@ __global__ __launch_bounds__(kNT) void k_b_beta(MultiBuffers B, int nb) {
   }
 }
 
+// x += alpha p with the p of this iteration, then p = D^-1 r + beta p.  The
+// iteration that finishes the solve returns here before the x update (done is
+// set); k_b_x_finish applies it once after the loop.
 template <int S>
 __global__ __launch_bounds__(kNT) void k_b_direction(MultiBuffers B) {
   MultiState<S>* st = static_cast<MultiState<S>*>(B.st);
   if (st->done) return;
-  __shared__ double s_beta[S];
-  if (threadIdx.x < S) s_beta[threadIdx.x] = st->beta[threadIdx.x];
+  __shared__ double s_alpha[S], s_beta[S];
+  if (threadIdx.x < S) {
+    s_alpha[threadIdx.x] = st->alpha[threadIdx.x];
+    s_beta[threadIdx.x] = st->beta[threadIdx.x];
+  }
   __syncthreads();
   for_group_rows<S>(B.n, [&](int64_t i, int k) {
     const int64_t o = i * S + k;
-    B.p[o] = B.z[o] + s_beta[k] * B.p[o];
+    const double pk = B.p[o];
+    B.x[o] += s_alpha[k] * pk;
+    const double ri = B.r[o];
+    const double zi = B.dinv ? B.dinv[i] * ri : ri;
+    B.p[o] = zi + s_beta[k] * pk;
+  });
+}
+
+// the x update of the last iteration (alpha = 0 for columns finished before
+// it, and after the setup)
+template <int S>
+__global__ __launch_bounds__(kNT) void k_b_x_finish(MultiBuffers B) {
+  const MultiState<S>* st = static_cast<const MultiState<S>*>(B.st);
+  __shared__ double s_alpha[S];
+  if (threadIdx.x < S) s_alpha[threadIdx.x] = st->alpha[threadIdx.x];
+  __syncthreads();
+  for_group_rows<S>(B.n, [&](int64_t i, int k) {
+    const int64_t o = i * S + k;
+    B.x[o] += s_alpha[k] * B.p[o];
   });
 }
 
@@ -370,6 +394,12 @@ hipError_t iteration_s(hipStream_t s, const MultiBuffers& B) {
 }
 
 template <int S>
+hipError_t x_finish_s(hipStream_t s, const MultiBuffers& B) {
+  hipLaunchKernelGGL(k_b_x_finish<S>, dim3(B.nblocks), dim3(kNT), 0, s, B);
+  return hipGetLastError();
+}
+
+template <int S>
 hipError_t residual_s(hipStream_t s, const MultiBuffers& B, double* out) {
   hipError_t e;
   if ((e = launch_spmm<S>(s, B, B.x, B.q)) != hipSuccess) return e;
@@ -416,8 +446,8 @@ size_t multi_state_bytes(int S) {
 }
 
 int multi_spmm_lds(int lds_per_cu) {
-  // k_b_spmm_pq_bs holds kNT * kC doubles of LDS; 512 bytes of allocation slack
-  const int pad = lds_per_cu / 2 - int(kNT * kC * sizeof(double)) - 512;
+  // k_b_spmm_pq_bs holds kNT * kCB doubles of LDS; 512 bytes of allocation slack
+  const int pad = lds_per_cu / 2 - int(kNT * kCB * sizeof(double)) - 512;
   return pad > 0 ? pad : 0;
 }
 
@@ -470,6 +500,10 @@ hipError_t launch_multi_setup(hipStream_t s, const MultiBuffers& B, double rtol,
 
 hipError_t launch_multi_iteration(hipStream_t s, const MultiBuffers& B) {
   GF_MULTI_SWITCH(B.S, iteration_s, s, B)
+}
+
+hipError_t launch_multi_x_finish(hipStream_t s, const MultiBuffers& B) {
+  GF_MULTI_SWITCH(B.S, x_finish_s, s, B)
 }
 
 hipError_t launch_multi_residual(hipStream_t s, const MultiBuffers& B, double* out) {

@@ -21,7 +21,7 @@ struct MultiBuffers {
   const double* dinv;  // null: no preconditioner
   const double* b;     // [n][S]
   double* x;           // [n][S]
-  double *r, *z, *p, *q;
+  double *r, *p, *q;  // z = D^-1 r is formed where it is used
   double* part1;  // [S][nblocks]
   double* part2;  // [3S][nblocks]
   void* st;       // MultiState<S>
@@ -51,6 +51,8 @@ hipError_t launch_multi_unpack(hipStream_t s, int S, int64_t n, int ncols, const
                                double* cols);
 hipError_t launch_multi_setup(hipStream_t s, const MultiBuffers& B, double rtol, int maxit);
 hipError_t launch_multi_iteration(hipStream_t s, const MultiBuffers& B);
+// x += alpha p of the iteration that finished the loop (once, after it)
+hipError_t launch_multi_x_finish(hipStream_t s, const MultiBuffers& B);
 hipError_t launch_multi_residual(hipStream_t s, const MultiBuffers& B, double* out);
 hipError_t read_multi_state(hipStream_t s, const MultiBuffers& B, MultiHostState* h);
 

@@ -88,17 +88,23 @@ __device__ inline void reduce_columns(const double* part, int nb, int base, doub
 }
 
 constexpr int kC = 4;
+// load cases per lane in the FP64 block SpMM.  8 would halve the lanes that
+// receive each block's 72 bytes of values, but a workgroup pass then covers
+// twice the node rows, the rows in flight per XCD gather from twice the rows
+// of p, and the block iteration measured 8 % slower (Brick20 48^3, 32 cases).
+constexpr int kCB = 4;
 
-template <int S>
+// KC load cases per lane (kC, or 8 in the FP64 block SpMM)
+template <int S, int KC = kC>
 struct Chunks {
-  static constexpr int L = S / kC;   // lanes per row
+  static constexpr int L = S / KC;   // lanes per row
   static constexpr int G = kNT / L;  // rows per workgroup pass
 };
 
-template <int S>
+template <int S, int KC = kC>
 __device__ inline void chunk_lane(int* g, int* c0) {
-  *g = threadIdx.x / Chunks<S>::L;
-  *c0 = (threadIdx.x % Chunks<S>::L) * kC;
+  *g = threadIdx.x / Chunks<S, KC>::L;
+  *c0 = (threadIdx.x % Chunks<S, KC>::L) * KC;
 }
 
 using dv2 = double __attribute__((ext_vector_type(2)));
@@ -138,6 +144,36 @@ __device__ inline void store_chunk(float* p, const float* v) {
   a.z = v[2];
   a.w = v[3];
   *reinterpret_cast<fv4*>(p) = a;
+}
+
+// FP64 chunks in split order: lane l = c0 / KC of a row owns the KC/2 pairs
+// of load cases {2l, 2l + 1} + q S / (KC/2), q = 0 .. KC/2 - 1, so each
+// 16-byte load of the L lanes of a row reads one contiguous piece of the row
+// (one 128-byte line at S = 32, KC = 4) instead of parts of two lines.
+template <int S, int KC>
+__device__ inline int split_col(int c0, int j) {
+  return (j / 2) * (S / (KC / 2)) + 2 * (c0 / KC) + (j & 1);
+}
+
+template <int S, int KC>
+__device__ inline void load_split(const double* __restrict__ row, int c0, double* v) {
+#pragma unroll
+  for (int q = 0; q < KC / 2; ++q) {
+    const dv2 a = *reinterpret_cast<const dv2*>(row + split_col<S, KC>(c0, 2 * q));
+    v[2 * q] = a.x;
+    v[2 * q + 1] = a.y;
+  }
+}
+
+template <int S, int KC>
+__device__ inline void store_split(double* row, int c0, const double* v) {
+#pragma unroll
+  for (int q = 0; q < KC / 2; ++q) {
+    dv2 a;
+    a.x = v[2 * q];
+    a.y = v[2 * q + 1];
+    *reinterpret_cast<dv2*>(row + split_col<S, KC>(c0, 2 * q)) = a;
+  }
 }
 
 // the arrays an SpMM reads besides the values: ELL columns and row lengths,
@@ -201,55 +237,77 @@ __device__ inline void ell_row_chunk(const SpmmArgs& A, const VT* __restrict__ v
 // off-diagonal blocks multiply x_J - x_R, the diagonal slot (node column R)
 // holds the block row sum and multiplies x_R.
 // NB: blocks loaded per step (their loads in flight together); the blocks
-// are added in slot order whatever NB is.
-template <int S, int D, class T, class VT, bool DIFF = false, int NB = 2, class F>
+// are added in slot order whatever NB is.  SPLIT (FP64 only): the lanes hold
+// their KC load cases in split order (load_split), acc included.
+template <int S, int D, class T, class VT, bool DIFF = false, int NB = 2, bool SPLIT = false,
+          int KC = kC, class F>
 __device__ inline void for_chunk_node_rows(const SpmmArgs& A, const VT* __restrict__ val,
                                            const T* __restrict__ xin, F&& f) {
   int g, c0;
-  chunk_lane<S>(&g, &c0);
+  chunk_lane<S, KC>(&g, &c0);
   const int64_t npos = A.bs.nslices * 64;
-  const int64_t stride = int64_t(gridDim.x) * Chunks<S>::G;
-  for (int64_t pos = int64_t(logical_block()) * Chunks<S>::G + g; pos < npos; pos += stride) {
+  const int64_t stride = int64_t(gridDim.x) * Chunks<S, KC>::G;
+  for (int64_t pos = int64_t(logical_block()) * Chunks<S, KC>::G + g; pos < npos; pos += stride) {
     const int32_t R = A.bs.perm[pos];
     if (R < 0) continue;
     const int lp = int(pos & 63);
     const int32_t cnt = A.rownnz[int64_t(D) * R] / D;
     const int64_t b0 = A.bs.sptr[pos >> 6];
-    T acc[D][kC];
+    T acc[D][KC];
 #pragma unroll
     for (int r = 0; r < D; ++r)
 #pragma unroll
-      for (int j = 0; j < kC; ++j) acc[r][j] = T(0);
-    T xr[D][kC];
+      for (int j = 0; j < KC; ++j) acc[r][j] = T(0);
+    auto gather = [&](int64_t row, T* xv) {
+      if constexpr (SPLIT) load_split<S, KC>(xin + row * S, c0, xv);
+      else load_chunk(xin + row * S + c0, xv);
+    };
+    T xr[D][KC];
     if constexpr (DIFF) {
 #pragma unroll
       for (int e = 0; e < D; ++e) load_chunk(xin + (int64_t(D) * R + e) * S + c0, xr[e]);
     }
     // x_J - x_R off the diagonal (DIFF)
-    auto rel = [&](int64_t c, T (*xv)[kC]) {
+    auto rel = [&](int64_t c, T (*xv)[KC]) {
       if constexpr (DIFF) {
         if (c != int64_t(D) * R) {
 #pragma unroll
           for (int e = 0; e < D; ++e)
 #pragma unroll
-            for (int j = 0; j < kC; ++j) xv[e][j] -= xr[e][j];
+            for (int j = 0; j < KC; ++j) xv[e][j] -= xr[e][j];
         }
       }
     };
+    // The node columns of the next NB blocks are loaded one step ahead, so a
+    // step issues its gathers of x at once instead of first waiting for its
+    // columns; the look-ahead index is clamped to the row's last block (the
+    // extra load is never used).
+    int32_t cn[NB];
+    const int64_t last = b0 + (cnt > 0 ? cnt - 1 : 0);
+    auto col_at = [&](int32_t jn) {
+      const int64_t b = b0 + jn;
+      return A.col[(b < last ? b : last) * 64 + lp];
+    };
+    if (cnt > 0) {
+#pragma unroll
+      for (int u = 0; u < NB; ++u) cn[u] = col_at(u);
+    }
     int32_t jb = 0;
     for (; jb + NB <= cnt; jb += NB) {
-      T v[NB][D * D], xv[NB][D][kC];
+      T v[NB][D * D], xv[NB][D][KC];
       int64_t cu[NB];
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
         const int64_t b = b0 + jb + u;
-        const int64_t c = int64_t(A.col[b * 64 + lp]) * D;
+        const int64_t c = int64_t(cn[u]) * D;
         cu[u] = c;
 #pragma unroll
-        for (int k = 0; k < D * D; ++k) v[u][k] = T(val[bsell_index<VT>(D * D, b, k, lp)]);
+        for (int e = 0; e < D; ++e) gather(c + e, xv[u][e]);
 #pragma unroll
-        for (int e = 0; e < D; ++e) load_chunk(xin + (c + e) * S + c0, xv[u][e]);
+        for (int k = 0; k < D * D; ++k) v[u][k] = T(val[bsell_index<VT>(D * D, b, k, lp)]);
       }
+#pragma unroll
+      for (int u = 0; u < NB; ++u) cn[u] = col_at(jb + NB + u);
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
         rel(cu[u], xv[u]);
@@ -258,40 +316,45 @@ __device__ inline void for_chunk_node_rows(const SpmmArgs& A, const VT* __restri
 #pragma unroll
           for (int e = 0; e < D; ++e)
 #pragma unroll
-            for (int j = 0; j < kC; ++j) acc[r][j] += v[u][r * D + e] * xv[u][e][j];
+            for (int j = 0; j < KC; ++j) acc[r][j] += v[u][r * D + e] * xv[u][e][j];
       }
     }
-    for (; jb < cnt; ++jb) {
-      const int64_t b = b0 + jb;
-      const int64_t c = int64_t(A.col[b * 64 + lp]) * D;
-      T v[D * D], xv[D][kC];
+    // the last cnt - jb < NB blocks, whose columns are already in cn
 #pragma unroll
-      for (int k = 0; k < D * D; ++k) v[k] = T(val[bsell_index<VT>(D * D, b, k, lp)]);
+    for (int u = 0; u + 1 < NB; ++u) {
+      if (jb + u < cnt) {
+        const int64_t b = b0 + jb + u;
+        const int64_t c = int64_t(cn[u]) * D;
+        T v[D * D], xv[D][KC];
 #pragma unroll
-      for (int e = 0; e < D; ++e) load_chunk(xin + (c + e) * S + c0, xv[e]);
-      rel(c, xv);
+        for (int e = 0; e < D; ++e) gather(c + e, xv[e]);
 #pragma unroll
-      for (int r = 0; r < D; ++r)
+        for (int k = 0; k < D * D; ++k) v[k] = T(val[bsell_index<VT>(D * D, b, k, lp)]);
+        rel(c, xv);
 #pragma unroll
-        for (int e = 0; e < D; ++e)
+        for (int r = 0; r < D; ++r)
 #pragma unroll
-          for (int j = 0; j < kC; ++j) acc[r][j] += v[r * D + e] * xv[e][j];
+          for (int e = 0; e < D; ++e)
+#pragma unroll
+            for (int j = 0; j < KC; ++j) acc[r][j] += v[r * D + e] * xv[e][j];
+      }
     }
     f(R, c0, acc);
   }
 }
 
 // per-column workgroup sums of the lanes' chunk partials into part[k][block]
-template <int S>
+// (SPLIT: the partials are in split order)
+template <int S, bool SPLIT = false, int KC = kC>
 __device__ inline void chunk_partials(const double* v, double* sh, double* part) {
   int g, c0;
-  chunk_lane<S>(&g, &c0);
+  chunk_lane<S, KC>(&g, &c0);
 #pragma unroll
-  for (int j = 0; j < kC; ++j) sh[g * S + c0 + j] = v[j];
+  for (int j = 0; j < KC; ++j) sh[g * S + (SPLIT ? split_col<S, KC>(c0, j) : c0 + j)] = v[j];
   __syncthreads();
   if (threadIdx.x < S) {
     double t = 0.0;
-    for (int q = 0; q < Chunks<S>::G; ++q) t += sh[q * S + threadIdx.x];
+    for (int q = 0; q < Chunks<S, KC>::G; ++q) t += sh[q * S + threadIdx.x];
     part[int64_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;
   }
 }
