@@ -15,7 +15,7 @@ struct gf_mmulti_ws {
   int64_t n = 0;
   int S = 0;
   double *b = nullptr, *x = nullptr;
-  float *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *dinv32 = nullptr;
+  float *r = nullptr, *p = nullptr, *q = nullptr, *dinv32 = nullptr;
   double *part1 = nullptr, *part2 = nullptr, *part3 = nullptr;
   double* hist = nullptr;  // alpha, beta per column and iteration (kLanczosIters)
   void* st = nullptr;
@@ -58,8 +58,8 @@ hipError_t ensure_graph(gf_ctx* ctx, gf_mmulti_ws* W, const MixedMultiBuffers& B
 void release_multi_mixed(gf_solve_plan* P) {
   gf_mmulti_ws* W = P->mmulti;
   if (!W) return;
-  void* v[] = {W->b,     W->x,     W->r,     W->z,     W->p,    W->q,
-               W->dinv32, W->part1, W->part2, W->part3, W->hist, W->st};
+  void* v[] = {W->b,      W->x,     W->r,     W->p,     W->q,    W->dinv32,
+               W->part1, W->part2, W->part3, W->hist, W->st};
   for (void* d : v)
     if (d) (void)hipFree(d);
   if (W->graph) (void)hipGraphExecDestroy(W->graph);
@@ -83,7 +83,7 @@ int run_multi_mixed(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, const dou
     const size_t cells = size_t(n) * size_t(S);
     hipError_t e = hipMalloc(&W->b, cells * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&W->x, cells * sizeof(double));
-    float** f[] = {&W->r, &W->z, &W->p, &W->q};
+    float** f[] = {&W->r, &W->p, &W->q};
     for (float** v : f)
       if (e == hipSuccess) e = hipMalloc(v, cells * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&W->dinv32, size_t(n > 0 ? n : 1) * sizeof(float));
@@ -126,7 +126,6 @@ int run_multi_mixed(gf_ctx* ctx, gf_ell* m, const gf_solve_opts& opts, const dou
   B.b = W->b;
   B.x = W->x;
   B.r = W->r;
-  B.z = W->z;
   B.p = W->p;
   B.q = W->q;
   B.part1 = W->part1;

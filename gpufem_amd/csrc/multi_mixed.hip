@@ -77,15 +77,11 @@ __global__ __launch_bounds__(kNT) void k_mm_refresh(MixedMultiBuffers B, int set
     if (!any) return;
     if (setup) {
       store_chunk(B.r + o, r32);
-      store_chunk(B.z + o, z32);
       store_chunk(B.p + o, z32);
     } else {
 #pragma unroll
       for (int j = 0; j < kC; ++j)
-        if (st->col_refresh[c0 + j]) {
-          B.r[o + j] = r32[j];
-          B.z[o + j] = z32[j];
-        }
+        if (st->col_refresh[c0 + j]) B.r[o + j] = r32[j];
     }
   };
   if constexpr (D == 0) {
@@ -233,7 +229,6 @@ __global__ __launch_bounds__(kNT) void k_mm_update(MixedMultiBuffers B) {
     const float r32 = float(double(B.r[o]) - a * double(B.q[o]));
     const float z32 = B.dinv32 ? B.dinv32[i] * r32 : r32;
     B.r[o] = r32;
-    B.z[o] = z32;
     rz += double(r32) * double(z32);
     rr += double(r32) * double(r32);
   });
@@ -342,9 +337,12 @@ __global__ __launch_bounds__(kNT) void k_mm_direction(MixedMultiBuffers B) {
   __shared__ double s_beta[S];
   if (threadIdx.x < S) s_beta[threadIdx.x] = st->beta[threadIdx.x];
   __syncthreads();
+  // z = D^-1 r formed here (the FP32 product the update and refresh use)
   for_group_rows<S>(B.n, [&](int64_t i, int k) {
     const int64_t o = i * S + k;
-    B.p[o] = float(double(B.z[o]) + s_beta[k] * double(B.p[o]));
+    const float r32 = B.r[o];
+    const float z32 = B.dinv32 ? B.dinv32[i] * r32 : r32;
+    B.p[o] = float(double(z32) + s_beta[k] * double(B.p[o]));
   });
 }
 

@@ -25,7 +25,7 @@ struct MixedMultiBuffers {
   const float* dinv32;  // null: no preconditioner
   const double* b;      // [n][S]
   double* x;            // [n][S]
-  float *r, *z, *p, *q;  // [n][S]
+  float *r, *p, *q;  // [n][S]; z = D^-1 r is formed where it is used
   double* part1;  // [S][nblocks]   p.q
   double* part2;  // [3S][nblocks]  r.z, r.r (and b.b at setup)
   double* part3;  // [2S][nblocks]  refreshed r.z, r.r
