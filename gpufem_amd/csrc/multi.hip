@@ -107,7 +107,8 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   if (st->done) return;
   __shared__ double sh[kNT * kCB];
   double pq[kCB] = {};
-  for_chunk_node_rows<S, D, double, double, false, NB, true, kCB>(
+  constexpr bool share = Chunks<S, kCB>::L >= NB * (D * D / 2);
+  for_chunk_node_rows<S, D, double, double, false, NB, true, kCB, share>(
       spmm_args(B), B.val, B.p, [&](int32_t R, int c0, const double (*acc)[kCB]) {
 #pragma unroll
         for (int r = 0; r < D; ++r) {

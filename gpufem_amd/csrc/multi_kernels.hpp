@@ -238,9 +238,13 @@ __device__ inline void ell_row_chunk(const SpmmArgs& A, const VT* __restrict__ v
 // holds the block row sum and multiplies x_R.
 // NB: blocks loaded per step (their loads in flight together); the blocks
 // are added in slot order whatever NB is.  SPLIT (FP64 only): the lanes hold
-// their KC load cases in split order (load_split), acc included.
+// their KC load cases in split order (load_split), acc included.  SHARE (FP64
+// only): the L lanes of a row load a step's block values once between them and
+// exchange them (a cross-lane permute through the LDS crossbar) instead of
+// each loading all of them; the vector-memory path then carries mostly the
+// gathers of x.
 template <int S, int D, class T, class VT, bool DIFF = false, int NB = 2, bool SPLIT = false,
-          int KC = kC, class F>
+          int KC = kC, bool SHARE = false, class F>
 __device__ inline void for_chunk_node_rows(const SpmmArgs& A, const VT* __restrict__ val,
                                            const T* __restrict__ xin, F&& f) {
   int g, c0;
@@ -303,8 +307,36 @@ __device__ inline void for_chunk_node_rows(const SpmmArgs& A, const VT* __restri
         cu[u] = c;
 #pragma unroll
         for (int e = 0; e < D; ++e) gather(c + e, xv[u][e]);
+        if constexpr (!SHARE) {
 #pragma unroll
-        for (int k = 0; k < D * D; ++k) v[u][k] = T(val[bsell_index<VT>(D * D, b, k, lp)]);
+          for (int k = 0; k < D * D; ++k) v[u][k] = T(val[bsell_index<VT>(D * D, b, k, lp)]);
+        }
+      }
+      if constexpr (SHARE) {
+        // lane ln of the row loads value pair ln % P of block ln / P (and the
+        // lanes ln < NB the odd last value of block ln); every lane of the
+        // row then takes the step's values from those lanes
+        constexpr int L = Chunks<S, KC>::L;
+        constexpr int P = D * D / 2;
+        static_assert(NB * P <= L, "a step's value pairs need a lane each");
+        const int ln = c0 / KC;
+        dv2 pr = {0.0, 0.0};
+        if (ln < NB * P)
+          pr = *reinterpret_cast<const dv2*>(
+              val + bsell_index<VT>(D * D, b0 + jb + ln / P, 2 * (ln % P), lp));
+        double sg = 0.0;
+        if constexpr ((D * D) & 1) {
+          if (ln < NB) sg = val[bsell_index<VT>(D * D, b0 + jb + ln, D * D - 1, lp)];
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+#pragma unroll
+          for (int q = 0; q < P; ++q) {
+            v[u][2 * q] = __shfl(pr.x, u * P + q, L);
+            v[u][2 * q + 1] = __shfl(pr.y, u * P + q, L);
+          }
+          if constexpr ((D * D) & 1) v[u][D * D - 1] = __shfl(sg, u, L);
+        }
       }
 #pragma unroll
       for (int u = 0; u < NB; ++u) cn[u] = col_at(jb + NB + u);
