@@ -238,11 +238,10 @@ __device__ inline void ell_row_chunk(const SpmmArgs& A, const VT* __restrict__ v
 // holds the block row sum and multiplies x_R.
 // NB: blocks loaded per step (their loads in flight together); the blocks
 // are added in slot order whatever NB is.  SPLIT (FP64 only): the lanes hold
-// their KC load cases in split order (load_split), acc included.  SHARE (FP64
-// only): the L lanes of a row load a step's block values once between them and
-// exchange them (a cross-lane permute through the LDS crossbar) instead of
-// each loading all of them; the vector-memory path then carries mostly the
-// gathers of x.
+// their KC load cases in split order (load_split), acc included.  SHARE: the L
+// lanes of a row load a step's block values once between them and exchange
+// them (a cross-lane permute through the LDS crossbar) instead of each loading
+// all of them; the vector-memory path then carries mostly the gathers of x.
 template <int S, int D, class T, class VT, bool DIFF = false, int NB = 2, bool SPLIT = false,
           int KC = kC, bool SHARE = false, class F>
 __device__ inline void for_chunk_node_rows(const SpmmArgs& A, const VT* __restrict__ val,
@@ -313,29 +312,32 @@ __device__ inline void for_chunk_node_rows(const SpmmArgs& A, const VT* __restri
         }
       }
       if constexpr (SHARE) {
-        // lane ln of the row loads value pair ln % P of block ln / P (and the
-        // lanes ln < NB the odd last value of block ln); every lane of the
-        // row then takes the step's values from those lanes
+        // lane ln of the row loads vector ln % P (a pair of doubles or a quad
+        // of floats) of block ln / P, and the lanes ln < NB the block's odd
+        // last value; every lane of the row then takes the step's values from
+        // those lanes
         constexpr int L = Chunks<S, KC>::L;
-        constexpr int P = D * D / 2;
-        static_assert(NB * P <= L, "a step's value pairs need a lane each");
+        constexpr int W = sizeof(VT) == 4 ? 4 : 2;
+        constexpr int P = D * D / W;
+        constexpr bool ODD = D * D - P * W == 1;
+        static_assert(NB * P <= L && D * D - P * W <= 1, "a step's value vectors need a lane each");
+        using VW = VT __attribute__((ext_vector_type(W)));
         const int ln = c0 / KC;
-        dv2 pr = {0.0, 0.0};
+        VW pr = {};
         if (ln < NB * P)
-          pr = *reinterpret_cast<const dv2*>(
-              val + bsell_index<VT>(D * D, b0 + jb + ln / P, 2 * (ln % P), lp));
-        double sg = 0.0;
-        if constexpr ((D * D) & 1) {
+          pr = *reinterpret_cast<const VW*>(
+              val + bsell_index<VT>(D * D, b0 + jb + ln / P, W * (ln % P), lp));
+        VT sg = VT(0);
+        if constexpr (ODD) {
           if (ln < NB) sg = val[bsell_index<VT>(D * D, b0 + jb + ln, D * D - 1, lp)];
         }
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
 #pragma unroll
-          for (int q = 0; q < P; ++q) {
-            v[u][2 * q] = __shfl(pr.x, u * P + q, L);
-            v[u][2 * q + 1] = __shfl(pr.y, u * P + q, L);
-          }
-          if constexpr ((D * D) & 1) v[u][D * D - 1] = __shfl(sg, u, L);
+          for (int q = 0; q < P; ++q)
+#pragma unroll
+            for (int w = 0; w < W; ++w) v[u][W * q + w] = T(__shfl(pr[w], u * P + q, L));
+          if constexpr (ODD) v[u][D * D - 1] = T(__shfl(sg, u, L));
         }
       }
 #pragma unroll

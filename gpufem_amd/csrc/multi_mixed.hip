@@ -169,11 +169,12 @@ __global__ __launch_bounds__(kNT) void k_mm_spmm_pq(MixedMultiBuffers B) {
       row(i, c0, acc);
     });
   } else {
-    for_chunk_node_rows<S, D, float, float, true>(spmm_args(B), B.val32, B.p,
-                              [&](int32_t R, int c0, const float (*acc)[kC]) {
+    constexpr bool share = Chunks<S>::L >= 2 * (D * D / 4);
+    for_chunk_node_rows<S, D, float, float, true, 2, false, kC, share>(
+        spmm_args(B), B.val32, B.p, [&](int32_t R, int c0, const float (*acc)[kC]) {
 #pragma unroll
-                                for (int r = 0; r < D; ++r) row(int64_t(D) * R + r, c0, acc[r]);
-                              });
+          for (int r = 0; r < D; ++r) row(int64_t(D) * R + r, c0, acc[r]);
+        });
   }
   chunk_partials<S>(pq, sh, B.part1);
 }
