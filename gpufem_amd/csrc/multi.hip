@@ -97,11 +97,11 @@ __global__ __launch_bounds__(kNT) void k_b_init_finish(MultiBuffers B, int nb, i
   }
 }
 
-// q = K p and the per-column partials of p.q.  Launched at two workgroups per
-// CU (padded LDS), i.e. two waves per SIMD, so the compiler may spend up to
-// 256 registers keeping every load of a step in flight.
+// q = K p and the per-column partials of p.q.  Launched at three workgroups
+// per CU (padded LDS), i.e. three waves per SIMD, so the compiler may spend up
+// to 168 registers keeping every load of a step in flight.
 template <int S, int D, int NB = 2>
-__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_b_spmm_pq_bs(
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_b_spmm_pq_bs(
     MultiBuffers B) {
   const MultiState<S>* st = static_cast<const MultiState<S>*>(B.st);
   if (st->done) return;
@@ -349,8 +349,8 @@ inline int grid_for(int64_t work, int threads, int cap) {
 template <int S>
 hipError_t launch_spmm_pq(hipStream_t s, const MultiBuffers& B) {
   const dim3 g(B.nblocks), t(kNT);
-  // padded LDS: two workgroups per CU keep each XCD's rows in flight (and the
-  // 32-wide rows of p they gather) closer to what its L2 holds
+  // padded LDS: three workgroups per CU (four fit) keep each XCD's rows in
+  // flight (and the 32-wide rows of p they gather) closer to what its L2 holds
   if (B.bsell_d == 2) hipLaunchKernelGGL((k_b_spmm_pq_bs<S, 2>), g, t, B.spmm_lds, s, B);
   else if (B.bsell_d == 3) hipLaunchKernelGGL((k_b_spmm_pq_bs<S, 3>), g, t, B.spmm_lds, s, B);
   else hipLaunchKernelGGL(k_b_spmm_pq<S>, g, t, 0, s, B);
@@ -447,8 +447,11 @@ size_t multi_state_bytes(int S) {
 }
 
 int multi_spmm_lds(int lds_per_cu) {
-  // k_b_spmm_pq_bs holds kNT * kCB doubles of LDS; 512 bytes of allocation slack
-  const int pad = lds_per_cu / 2 - int(kNT * kCB * sizeof(double)) - 512;
+  // k_b_spmm_pq_bs holds kNT * kCB doubles of LDS; 512 bytes of allocation
+  // slack.  Three per CU measured 2 % faster per block iteration than two
+  // (Brick20 48^3, 32 load cases); four made an earlier form of the kernel
+  // 1.4x slower per launch than two (more rows of p in flight, L2 misses).
+  const int pad = lds_per_cu / 3 - int(kNT * kCB * sizeof(double)) - 512;
   return pad > 0 ? pad : 0;
 }
 

@@ -28,7 +28,7 @@ struct MultiBuffers {
   // block-sliced copy (bsell_d = 2 or 3: val/col are its arrays; 0: ELL)
   int bsell_d;
   BsellView bs;
-  // extra LDS bytes per SpMM workgroup (multi_spmm_lds): at most two
+  // extra LDS bytes per SpMM workgroup (multi_spmm_lds): at most three
   // workgroups share a CU
   int spmm_lds;
 };
@@ -42,7 +42,7 @@ struct MultiHostState {
 int multi_width(int ncols);       // smallest supported S >= ncols (0: too many)
 size_t multi_state_bytes(int S);
 int multi_blocks(int64_t n, int S);
-// Dynamic LDS that leaves room for two block-sliced SpMM workgroups on a CU
+// Dynamic LDS that leaves room for three block-sliced SpMM workgroups on a CU
 // with lds_per_cu bytes (0: no padding).
 int multi_spmm_lds(int lds_per_cu);
 hipError_t launch_multi_pack(hipStream_t s, int S, int64_t n, int ncols, const double* cols,

@@ -107,10 +107,16 @@ def summarize(paths):
     for path in paths:
         db = sqlite3.connect(path)
         print("== %s" % path)
-        rows = db.execute("select name, counter_name, counter_value from pmc_events").fetchall()
+        rows = db.execute("select name, counter_name, dispatch_id, counter_value "
+                          "from pmc_events").fetchall()
         if rows:
+            # a counter with several hardware instances (shader engines, XCDs)
+            # has one row per instance: summed per dispatch, then averaged
+            per = collections.defaultdict(float)
+            for name, cn, d, v in rows:
+                per[(name, cn, d)] += v
             agg = collections.defaultdict(list)
-            for name, cn, v in rows:
+            for (name, cn, d), v in per.items():
                 agg[(name, cn)].append(v)
             print("| kernel | counter | dispatches | mean per dispatch |")
             print("|---|---|---|---|")
