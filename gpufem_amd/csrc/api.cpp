@@ -500,11 +500,6 @@ static bool gather_fits(const gf_mesh* mesh, const gf_ell* m) {
   return cap > 0 && m->width / m->dim <= cap && mesh->nelem < (int64_t(1) << 26);
 }
 
-// the two-phase kernel: every node fits its slots
-static bool pairs_fits(const gf_mesh* mesh, const gf_ell* m) {
-  const int cap = pairs_max_slots(mesh->type);
-  return cap > 0 && m->width / m->dim <= cap && mesh->nelem < (int64_t(1) << 26);
-}
 
 static int assemble_with(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int variant, gf_timing* t) {
   const double h0 = now_ms();
@@ -518,7 +513,7 @@ static int assemble_with(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int varian
     GF_HIP_CHECK(ctx, hipMalloc(&m->d_etens, size_t(std::max<int64_t>(cnt, 1)) * sizeof(double)));
   }
   if (variant == 4 && !m->d_egrad) {
-    const int64_t cnt = pairs_egrad_doubles(mesh->type, mesh->ngauss, mesh->nelem);
+    const int64_t cnt = gauss_geom_doubles(mesh->type, mesh->ngauss, mesh->nelem);
     GF_HIP_CHECK(ctx, hipMalloc(&m->d_egrad, size_t(std::max<int64_t>(cnt, 1)) * sizeof(double)));
   }
   int* d_bad = reinterpret_cast<int*>(static_cast<char*>(ctx->d_state) + 128);
@@ -526,7 +521,7 @@ static int assemble_with(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int varian
   int st = record_start(ctx);
   if (st) return st;
   const size_t slots = size_t(m->n) * size_t(m->width);
-  // the gathers and the two-phase kernel write every live slot; the scatters
+  // the gathers write every live slot; the scatters
   // accumulate from zero
   if (!node_rows)
     GF_HIP_CHECK(ctx, hipMemsetAsync(m->d_val, 0, slots * sizeof(double), ctx->stream));
@@ -556,10 +551,9 @@ static int assemble_with(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int varian
   // values (what the penalty and the Jacobi preconditioner read); the other
   // slots are materialised from the copy when something reads them
   a.diag = m->d_diag;
-  a.ell_diag_only = bsell && (variant == 2 || variant == 3);
+  a.ell_diag_only = bsell && node_rows;
   a.bsptr = m->d_bsptr;
   a.binv = m->d_binv;
-  a.col = m->d_col;
   a.nelem = mesh->nelem;
   int64_t launches = node_rows ? 0 : 1;
   if (variant == 2) {
@@ -569,7 +563,7 @@ static int assemble_with(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int varian
     GF_HIP_CHECK(ctx, launch_assembly_gather(ctx->stream, a));
     ++launches;
   } else if (variant == 4) {
-    GF_HIP_CHECK(ctx, launch_assembly_pairs(ctx->stream, a, m->d_egrad));
+    GF_HIP_CHECK(ctx, launch_assembly_geom(ctx->stream, a, m->d_egrad));
     launches += 2;
   } else {
     for (int c = 0; c < m->ncolors; ++c) {
@@ -623,14 +617,14 @@ int gf_assemble_tune(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, int32_t force,
   if (force == 3 && !gather_fits(mesh, m))
     return set_error(ctx, GF_ERR_UNSUPPORTED,
                      "gf_assemble_tune: the node-gather kernel does not fit this mesh");
-  if (force == 4 && !pairs_fits(mesh, m))
+  if (force == 4 && !gather_fits(mesh, m))
     return set_error(ctx, GF_ERR_UNSUPPORTED,
-                     "gf_assemble_tune: the two-phase kernel does not fit this mesh");
+                     "gf_assemble_tune: the stored-geometry gather does not fit this mesh");
   if (ms_per_variant)
     for (int v = 0; v < GF_ASM_VARIANTS; ++v) ms_per_variant[v] = -1.0;
   if (force >= 0) {
     m->asm_variant = force;
-    if (force != 4 && m->d_egrad) {  // the two-phase kernel's buffer is no longer needed
+    if (force != 4 && m->d_egrad) {  // the stored geometry is no longer needed
       GF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
       GF_HIP_CHECK(ctx, hipFree(m->d_egrad));
       m->d_egrad = nullptr;

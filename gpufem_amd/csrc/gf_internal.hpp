@@ -106,7 +106,8 @@ struct gf_ell {
   // assembly; lazily built)
   int64_t* d_n2e_ptr = nullptr;
   int32_t* d_n2e = nullptr;
-  // element gradients at the Gauss points (two-phase assembly; lazily built)
+  // J^-1 and w detJ E of every element at its Gauss points (assembly
+  // variant 4; lazily built)
   double* d_egrad = nullptr;
   double* d_etens = nullptr;  // element tensors of the matrix-core assembly (variant 2)
   std::vector<int32_t> h_elem_color;  // [nelem], 1-based colors

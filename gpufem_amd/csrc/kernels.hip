@@ -444,6 +444,9 @@ struct GArgs {
   int ell_diag_only;
   // element tensors (variant 2): tens[e][a][p][b][q]
   const double* tens;
+  // Gauss-point geometry (variant 4): per element and Gauss point, J^-1
+  // (DIM x DIM, row-major) and w detJ E
+  const double* geom;
 };
 
 // A unit is RPT rows of one node (UPN units per node).  LPU lanes share a
@@ -459,12 +462,16 @@ struct GArgs {
 // at a time (the Gauss-point geometry is recomputed per chunk), which bounds
 // the accumulator registers.  With RPT < DIM (one row per unit) the row's
 // entries are accumulated directly and BCH = NEN.  GPU: unroll factor of the
-// Gauss-point loop.
+// Gauss-point loop.  SRC: 0 forms the geometry from the coordinates, 1 reads
+// the element tensors, 2 reads J^-1 and w detJ E per Gauss point (formed once
+// per element by k_gauss_geom with the same arithmetic, so the values equal
+// SRC 0's bitwise).
 template <int DIM, int NEN, int NGP, int MAXS, int RPT, int LPU, int UPW, int BCH, int GPU = 1,
           int SRC = 0>
 __global__ __launch_bounds__(64) void k_assemble_gather(GArgs ga) {
   static_assert(LPU * UPW <= 64, "a workgroup is one wavefront");
-  static_assert(SRC == 0 || (RPT == DIM && DIM == 3), "element tensors: 3D node units");
+  static_assert(SRC != 1 || (RPT == DIM && DIM == 3), "element tensors: 3D node units");
+  static_assert(SRC != 2 || RPT == DIM, "stored geometry: node units");
   static_assert(NEN % BCH == 0 && (RPT == DIM || BCH == NEN), "neighbour chunks");
   constexpr int UPN = DIM / RPT;  // units per node
   constexpr int ACC = MAXS * RPT * DIM;
@@ -526,30 +533,44 @@ __global__ __launch_bounds__(64) void k_assemble_gather(GArgs ga) {
               for (int q = 0; q < 3; ++q) kl[b][p][q] = tt[(p * NEN + b) * 3 + q];
         }
 #pragma unroll(GPU)
-        for (int g = 0; g < (SRC == 0 ? NGP : 0); ++g) {
+        for (int g = 0; g < (SRC != 1 ? NGP : 0); ++g) {
           const double* dN = c_tab + a.tab + g * DIM * NEN;
-          double J[DIM][DIM];
+          double Ji[DIM][DIM];
+          double sc;
+          if constexpr (SRC == 0) {
+            double J[DIM][DIM];
 #pragma unroll
-          for (int i = 0; i < DIM; ++i)
+            for (int i = 0; i < DIM; ++i)
 #pragma unroll
-            for (int j = 0; j < DIM; ++j) {
+              for (int j = 0; j < DIM; ++j) {
+                double acc = 0.0;
+#pragma unroll
+                for (int n = 0; n < NEN; ++n) acc += dN[i * NEN + n] * x[n][j];
+                J[i][j] = acc;
+              }
+            const double det = det_inv<DIM>(J, Ji);
+            if (!(det > 0.0)) *a.bad = 1;
+            sc = c_tab[a.tab + NGP * DIM * NEN + g] * det * Ee;
+          } else {
+            const double* gp = ga.geom + (e * NGP + g) * (DIM * DIM + 1);
+#pragma unroll
+            for (int i = 0; i < DIM; ++i)
+#pragma unroll
+              for (int j = 0; j < DIM; ++j) Ji[i][j] = gp[i * DIM + j];
+            sc = gp[DIM * DIM];
+          }
+          // dN_n/dx at this Gauss point
+          auto grad = [&](int n, double* gv) {
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) {
               double acc = 0.0;
 #pragma unroll
-              for (int n = 0; n < NEN; ++n) acc += dN[i * NEN + n] * x[n][j];
-              J[i][j] = acc;
+              for (int k = 0; k < DIM; ++k) acc += Ji[i][k] * dN[k * NEN + n];
+              gv[i] = acc;
             }
-          double Ji[DIM][DIM];
-          const double det = det_inv<DIM>(J, Ji);
-          if (!(det > 0.0)) *a.bad = 1;
-          const double sc = c_tab[a.tab + NGP * DIM * NEN + g] * det * Ee;
+          };
           double gA[DIM];
-#pragma unroll
-          for (int i = 0; i < DIM; ++i) {
-            double acc = 0.0;
-#pragma unroll
-            for (int k = 0; k < DIM; ++k) acc += Ji[i][k] * dN[k * NEN + la];
-            gA[i] = acc;
-          }
+          grad(la, gA);
           if constexpr (RPT == DIM) {
             // every row of the node: accumulate the products T_pq; the
             // elastic combinations are formed once per element below
@@ -559,13 +580,7 @@ __global__ __launch_bounds__(64) void k_assemble_gather(GArgs ga) {
 #pragma unroll
             for (int b = 0; b < BCH; ++b) {
               double gB[DIM];
-#pragma unroll
-              for (int i = 0; i < DIM; ++i) {
-                double acc = 0.0;
-#pragma unroll
-                for (int k = 0; k < DIM; ++k) acc += Ji[i][k] * dN[k * NEN + c0 + b];
-                gB[i] = acc;
-              }
+              grad(c0 + b, gB);
 #pragma unroll
               for (int p = 0; p < DIM; ++p)
 #pragma unroll
@@ -575,13 +590,7 @@ __global__ __launch_bounds__(64) void k_assemble_gather(GArgs ga) {
 #pragma unroll
             for (int b = 0; b < BCH; ++b) {
               double gB[DIM];
-#pragma unroll
-              for (int i = 0; i < DIM; ++i) {
-                double acc = 0.0;
-#pragma unroll
-                for (int k = 0; k < DIM; ++k) acc += Ji[i][k] * dN[k * NEN + c0 + b];
-                gB[i] = acc;
-              }
+              grad(c0 + b, gB);
               double dot = 0.0;
 #pragma unroll
               for (int i = 0; i < DIM; ++i) dot += gA[i] * gB[i];
@@ -980,43 +989,20 @@ hipError_t launch_assembly_color(hipStream_t s, const AssemblyArgs& a) {
 }
 
 // ---------------------------------------------------------------------------
-// Two-phase assembly (variant 4).
-//
-// Phase 1, one thread per (element, Gauss point): J, det J, J^-1 and the
-// physical gradients of the element's nen shape functions, G[a][i] =
-// sum_k Jinv[i][k] dN_a/dxi_k, and the scale w_g det J E_e, written once to
-// egrad[(e ngp + g) kGS ...] (kGS = nen dim + 1 doubles).
-//
-// Phase 2, one thread per (node v, neighbour slot s), node-major: the d x d
-// block K_vw (w = the slot's node) summed over the elements that contain
-// both nodes (in ascending element order) and their Gauss points,
-//   K_vw[i][j] = sum_e sum_g sc (G_a . D . G_b)[i][j]
-// with the isotropic / plane-stress D written out (no B zeros multiplied),
-// kept in registers and written once into the ELL values (and the
-// block-sliced copy when it exists).  Each element's geometry is computed
-// once (the node gather recomputes it for every row of each of its nodes),
-// no coloring, no read-modify-write, and no LDS: the occupancy is set by
-// registers alone.
+// Gauss-point geometry of every element, once (the first kernel of variant
+// 4): one thread per (element, Gauss point) forms J, detJ and J^-1 from the
+// coordinates with the node gather's own arithmetic and stores J^-1
+// (row-major) and w detJ E; the node gather (SRC 2) then reads them instead
+// of forming them again for every node of the element and chunk of
+// neighbours.  Measured slower than forming them (Brick8 128^3 4.83 against
+// 4.15-4.3 ms, Q8 1024x512 0.70 against 0.58 ms; storing dN/dx of every node
+// instead: 5.9-6.3 and 1.02 ms): the gather is bound by its loads, not by
+// the FP64 work of the geometry.
 // ---------------------------------------------------------------------------
-struct PArgs {
-  KArgs k;
-  double* egrad;
-  int64_t nelem;
-  const int64_t* n2e_ptr;
-  const int32_t* n2e;
-  const int32_t* rownnz;
-  const int32_t* col;
-  int64_t nnodes;
-  double* bval;
-  const int64_t* bsptr;
-  const int32_t* binv;
-};
-
 template <int DIM, int NEN, int NGP>
-__global__ __launch_bounds__(256) void k_elem_gradients(PArgs pa) {
-  constexpr int GS = NEN * DIM + 1;
-  const KArgs& a = pa.k;
-  const int64_t units = pa.nelem * NGP;
+__global__ __launch_bounds__(256) void k_gauss_geom(KArgs a, double* geom, int64_t nelem) {
+  constexpr int GS = DIM * DIM + 1;
+  const int64_t units = nelem * NGP;
   for (int64_t u = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; u < units;
        u += int64_t(gridDim.x) * blockDim.x) {
     const int64_t e = u / NGP;
@@ -1043,92 +1029,12 @@ __global__ __launch_bounds__(256) void k_elem_gradients(PArgs pa) {
     const double det = det_inv<DIM>(J, Ji);
     if (!(det > 0.0)) *a.bad = 1;
     const double Ee = a.elemE ? a.elemE[e] : a.E;
-    double* out = pa.egrad + u * GS;
-#pragma unroll
-    for (int n = 0; n < NEN; ++n)
-#pragma unroll
-      for (int i = 0; i < DIM; ++i) {
-        double acc = 0.0;
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) acc += Ji[i][k] * dN[k * NEN + n];
-        out[n * DIM + i] = acc;
-      }
-    out[NEN * DIM] = c_tab[a.tab + NGP * DIM * NEN + g] * det * Ee;
-  }
-}
-
-template <int DIM, int NEN, int NGP, int MAXS>
-__global__ __launch_bounds__(256) void k_assemble_pairs(PArgs pa) {
-  constexpr int GS = NEN * DIM + 1;
-  const KArgs& a = pa.k;
-  const int64_t units = pa.nnodes * MAXS;
-  for (int64_t u = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; u < units;
-       u += int64_t(gridDim.x) * blockDim.x) {
-    const int64_t v = u / MAXS;
-    const int s = int(u - v * MAXS);
-    const int64_t row0 = v * DIM;
-    if (s >= pa.rownnz[row0] / DIM) continue;
-    const int32_t w = pa.col[row0 + int64_t(s) * DIM * a.n] / DIM;
-    double acc[DIM][DIM];
+    double* out = geom + u * GS;
 #pragma unroll
     for (int i = 0; i < DIM; ++i)
 #pragma unroll
-      for (int j = 0; j < DIM; ++j) acc[i][j] = 0.0;
-    const int64_t k1 = pa.n2e_ptr[v + 1];
-    for (int64_t kk = pa.n2e_ptr[v]; kk < k1; ++kk) {
-      const int32_t packed = pa.n2e[kk];
-      const int64_t e = packed >> 5;
-      const int la = packed & 31;
-      int lb = -1;
-#pragma unroll
-      for (int q = 0; q < NEN; ++q)
-        if (a.conn[e * NEN + q] - 1 == w) lb = q;
-      if (lb < 0) continue;
-      const double* ge = pa.egrad + e * NGP * GS;
-      for (int g = 0; g < NGP; ++g) {
-        const double* gp = ge + g * GS;
-        double gA[DIM], gB[DIM];
-#pragma unroll
-        for (int i = 0; i < DIM; ++i) {
-          gA[i] = gp[la * DIM + i];
-          gB[i] = gp[lb * DIM + i];
-        }
-        const double sc = gp[NEN * DIM];
-        if constexpr (DIM == 2) {
-          // plane stress: K_ii = c0 gA_i gB_i + c2 gA_k gB_k, K_ij = c1 gA_i gB_j + c2 gA_j gB_i
-          const double t00 = sc * gA[0] * gB[0], t11 = sc * gA[1] * gB[1];
-          const double t01 = sc * gA[0] * gB[1], t10 = sc * gA[1] * gB[0];
-          acc[0][0] += a.c0 * t00 + a.c2 * t11;
-          acc[1][1] += a.c0 * t11 + a.c2 * t00;
-          acc[0][1] += a.c1 * t01 + a.c2 * t10;
-          acc[1][0] += a.c1 * t10 + a.c2 * t01;
-        } else {
-          const double dot = gA[0] * gB[0] + gA[1] * gB[1] + gA[2] * gB[2];
-#pragma unroll
-          for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-              const double kij = i == j ? a.c0 * gA[i] * gB[i] + a.c2 * (dot - gA[i] * gB[i])
-                                        : a.c1 * gA[i] * gB[j] + a.c2 * gA[j] * gB[i];
-              acc[i][j] += sc * kij;
-            }
-        }
-      }
-    }
-    int64_t bb = 0;
-    int bl = 0;
-    if (pa.bval) {
-      const int32_t pos = pa.binv[v];
-      bb = pa.bsptr[pos >> 6] + s;
-      bl = pos & 63;
-    }
-#pragma unroll
-    for (int i = 0; i < DIM; ++i)
-#pragma unroll
-      for (int j = 0; j < DIM; ++j) {
-        a.val[row0 + i + (int64_t(s) * DIM + j) * a.n] = acc[i][j];
-        if (pa.bval) pa.bval[bsell_vidx(DIM * DIM, bb, i * DIM + j, bl)] = acc[i][j];
-      }
+      for (int j = 0; j < DIM; ++j) out[i * DIM + j] = Ji[i][j];
+    out[GS - 1] = c_tab[a.tab + NGP * DIM * NEN + g] * det * Ee;
   }
 }
 
@@ -1164,81 +1070,6 @@ hipError_t launch_gather2(hipStream_t s, const GArgs& g) {
   else return launch_gather_cfg<2, NEN, NGP, MAXS, 2, 2, 30, NEN>(s, g);
 }
 }  // namespace
-
-namespace {
-template <int DIM, int NEN, int NGP, int MAXS>
-hipError_t launch_pairs(hipStream_t s, const PArgs& p) {
-  if (p.nelem > 0) {
-    hipLaunchKernelGGL((k_elem_gradients<DIM, NEN, NGP>),
-                       dim3(grid_for(p.nelem * NGP, 256, 65536)), dim3(256), 0, s, p);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  if (p.nnodes == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_assemble_pairs<DIM, NEN, NGP, MAXS>),
-                     dim3(grid_for(p.nnodes * MAXS, 256, 1 << 20)), dim3(256), 0, s, p);
-  return hipGetLastError();
-}
-}  // namespace
-
-int pairs_max_slots(int type) {
-  switch (type) {
-    case GF_Q4: return 9;
-    case GF_Q8: return 21;
-    case GF_BRICK8: return 27;
-    case GF_BRICK20: return 81;
-    default: return 0;
-  }
-}
-
-int64_t pairs_egrad_doubles(int type, int ngauss, int64_t nelem) {
-  ElemInfo ei;
-  if (!elem_info(type, &ei)) return 0;
-  int ngp = 1;
-  for (int d = 0; d < ei.dim; ++d) ngp *= ngauss;
-  return nelem * ngp * (int64_t(ei.nen) * ei.dim + 1);
-}
-
-hipError_t launch_assembly_pairs(hipStream_t s, const AssemblyArgs& a, double* egrad) {
-  ElemInfo ei;
-  if (!elem_info(a.type, &ei) || (a.ngauss != 2 && a.ngauss != 3)) return hipErrorInvalidValue;
-  PArgs p;
-  KArgs& k = p.k;
-  k.conn = a.conn;
-  k.coords = a.coords;
-  k.elemE = a.elemE;
-  k.E = a.E;
-  k.c0 = a.c[0];
-  k.c1 = a.c[1];
-  k.c2 = a.c[2];
-  k.elems = nullptr;
-  k.nce = 0;
-  k.slot = nullptr;
-  k.val = a.val;
-  k.n = a.n;
-  k.bad = a.bad_jac;
-  k.tab = h_off[tab_block(a.type, a.ngauss)];
-  p.egrad = egrad;
-  p.nelem = a.nelem;
-  p.n2e_ptr = a.n2e_ptr;
-  p.n2e = a.n2e;
-  p.rownnz = a.rownnz;
-  p.col = a.col;
-  p.nnodes = a.nnodes;
-  p.bval = a.bval;
-  p.bsptr = a.bsptr;
-  p.binv = a.binv;
-  const bool g3 = a.ngauss == 3;
-  switch (a.type) {
-    case GF_Q4: return g3 ? launch_pairs<2, 4, 9, 9>(s, p) : launch_pairs<2, 4, 4, 9>(s, p);
-    case GF_Q8: return g3 ? launch_pairs<2, 8, 9, 21>(s, p) : launch_pairs<2, 8, 4, 21>(s, p);
-    case GF_BRICK8:
-      return g3 ? launch_pairs<3, 8, 27, 27>(s, p) : launch_pairs<3, 8, 8, 27>(s, p);
-    case GF_BRICK20:
-      return g3 ? launch_pairs<3, 20, 27, 81>(s, p) : launch_pairs<3, 20, 8, 81>(s, p);
-    default: return hipErrorInvalidValue;
-  }
-}
 
 int gather_max_slots(int type) {
   switch (type) {
@@ -1278,6 +1109,7 @@ GArgs gather_args(const AssemblyArgs& a) {
   g.diag = a.diag;
   g.ell_diag_only = a.bval && a.ell_diag_only ? 1 : 0;
   g.tens = nullptr;
+  g.geom = nullptr;
   return g;
 }
 
@@ -1314,6 +1146,52 @@ hipError_t launch_assembly_tensors(hipStream_t s, const AssemblyArgs& a, double*
       // chunks of 4/5: 4.22-4.23, 16 lanes x 4 nodes: 4.40-4.49, 4 x 8: 4.46)
       return g3 ? launch_tensors<20, 27, 81, 8, 8, 10>(s, a, tens)
                 : launch_tensors<20, 8, 81, 8, 8, 10>(s, a, tens);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int64_t gauss_geom_doubles(int type, int ngauss, int64_t nelem) {
+  ElemInfo ei;
+  if (!elem_info(type, &ei)) return 0;
+  int ngp = 1;
+  for (int d = 0; d < ei.dim; ++d) ngp *= ngauss;
+  return nelem * ngp * (int64_t(ei.dim) * ei.dim + 1);
+}
+
+namespace {
+template <int DIM, int NEN, int NGP, int MAXS, int LPU, int UPW, int BCH, int GPU>
+hipError_t launch_geom(hipStream_t s, const AssemblyArgs& a, double* geom) {
+  GArgs g = gather_args(a);
+  g.geom = geom;
+  if (a.nelem > 0) {
+    hipLaunchKernelGGL((k_gauss_geom<DIM, NEN, NGP>),
+                       dim3(grid_for(a.nelem * NGP, 256, 65536)), dim3(256), 0, s, g.k, geom,
+                       a.nelem);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return launch_gather_cfg<DIM, NEN, NGP, MAXS, DIM, LPU, UPW, BCH, GPU, 2>(s, g);
+}
+}  // namespace
+
+hipError_t launch_assembly_geom(hipStream_t s, const AssemblyArgs& a, double* geom) {
+  ElemInfo ei;
+  if (!elem_info(a.type, &ei) || (a.ngauss != 2 && a.ngauss != 3) || gather_max_slots(a.type) == 0)
+    return hipErrorInvalidValue;
+  const bool g3 = a.ngauss == 3;
+  switch (a.type) {
+    case GF_Q4:
+      return g3 ? launch_geom<2, 4, 9, 9, 2, 32, 4, 1>(s, a, geom)
+                : launch_geom<2, 4, 4, 9, 2, 32, 4, 1>(s, a, geom);
+    case GF_Q8:
+      return g3 ? launch_geom<2, 8, 9, 21, 2, 30, 8, 1>(s, a, geom)
+                : launch_geom<2, 8, 4, 21, 2, 30, 8, 1>(s, a, geom);
+    case GF_BRICK8:
+      return g3 ? launch_geom<3, 8, 27, 27, 4, 16, 4, 2>(s, a, geom)
+                : launch_geom<3, 8, 8, 27, 4, 16, 4, 2>(s, a, geom);
+    case GF_BRICK20:
+      return g3 ? launch_geom<3, 20, 27, 81, 8, 8, 10, 1>(s, a, geom)
+                : launch_geom<3, 20, 8, 81, 8, 8, 10, 1>(s, a, geom);
     default: return hipErrorInvalidValue;
   }
 }

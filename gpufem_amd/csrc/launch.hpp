@@ -47,18 +47,15 @@ struct AssemblyArgs {
   // (slot diag[row]) of the ELL values
   const int32_t* diag;
   int ell_diag_only;
-  // two-phase assembly: ELL columns and the element count
-  const int32_t* col;
+  // the element count (the first kernels of variants 2 and 4)
   int64_t nelem;
 };
 hipError_t launch_assembly_color(hipStream_t s, const AssemblyArgs& a);
-// Two-phase assembly (variant 4): the element gradients at the Gauss points
-// into egrad (pairs_egrad_doubles doubles), then one thread per (node,
-// neighbour slot) summing that block over the shared elements; applicable
-// when every node has at most pairs_max_slots(type) neighbours.
-int pairs_max_slots(int type);
-int64_t pairs_egrad_doubles(int type, int ngauss, int64_t nelem);
-hipError_t launch_assembly_pairs(hipStream_t s, const AssemblyArgs& a, double* egrad);
+// Variant 4: the Gauss-point geometry of every element once into geom (a
+// buffer of gauss_geom_doubles(type, ngauss, nelem) doubles), then the node
+// gather reading it instead of forming it per node; where the gather fits.
+int64_t gauss_geom_doubles(int type, int ngauss, int64_t nelem);
+hipError_t launch_assembly_geom(hipStream_t s, const AssemblyArgs& a, double* geom);
 // Node-gather assembly (variant 3): one launch for the whole mesh, no
 // coloring.  Each thread owns rows of one node, computes the rows of Ke of
 // every element around the node and adds them in LDS, then writes its rows

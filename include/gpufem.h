@@ -186,21 +186,21 @@ int gf_assemble(gf_ctx* ctx, const gf_mesh* mesh, gf_ell* m, gf_timing* t);
  * the node gather over them; 3 (at most 9 / 21 / 27 / 81 neighbours per
  * node for Q4 / Q8 / Brick8 / Brick20) = node gather, the lanes of a node
  * (or node row) forming its rows of every surrounding element's Ke and
- * writing them once (no coloring, no read-modify-write); 4 (forced only) =
- * two phases, the gradients of every element at its Gauss points computed
- * once into a buffer the matrix keeps while 4 is its variant (nelem x ngp x
- * (nen dim + 1) doubles), then one thread per (node, neighbour node)
- * summing that dim x dim block over the shared elements and writing it
- * once.  0 and 1 add the same terms in the same order (bitwise-identical
- * values); 2, 3 and 4 sum in other orders (equal to rounding).
+ * writing them once (no coloring, no read-modify-write); 4 (forced only,
+ * where 3 applies) = the Gauss-point geometry of every element (J^-1 and
+ * w detJ E) computed once into a buffer the matrix keeps while 4 is its
+ * variant (nelem x ngp x (dim^2 + 1) doubles), then the node gather of 3
+ * reading it instead of forming it per node.  0 and 1 add the same terms in
+ * the same order (bitwise-identical values), as do 3 and 4; 2 and 3 sum in
+ * other orders (equal to rounding).
  * force_variant 0..4 fixes it (GF_ERR_UNSUPPORTED where 2, 3 or 4 does not
  * fit); -1 assembles twice with each of 0, 1, 3 and, for Brick20 when its
  * tensors take at most half the free device memory, 2, and keeps the
  * fastest (gf_assemble does this on its first call for meshes of 65536
  * elements or more).  When the solver's block-sliced copy of the matrix
- * exists, 2, 3 and 4 write it (2 and 3 with only the diagonal of the ELL
- * values; the other ELL values are rebuilt from the copy by the first call
- * that reads them) and the others are followed by the copy, inside the
+ * exists, 2, 3 and 4 write it with only the diagonal of the ELL values
+ * (the other ELL values are rebuilt from the copy by the first call that
+ * reads them) and the others are followed by the copy, inside the
  * timing.  ms_per_variant (may be NULL): GF_ASM_VARIANTS entries, the better
  * of two timings of each variant timed, -1 otherwise. */
 #define GF_ASM_VARIANTS 5

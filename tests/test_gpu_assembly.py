@@ -149,19 +149,28 @@ def test_gather_assembly_matches_oracle(gpu_ctx, t, nx, ny, nz, ng):
 @pytest.mark.parametrize("t,nx,ny,nz,ng", GATHER_CASES + [(gf.GF_BRICK20, 4, 3, 3, 3),
                                                           (gf.GF_BRICK20, 3, 3, 2, 2)])
 def test_two_phase_assembly_matches_oracle(gpu_ctx, t, nx, ny, nz, ng):
-    """Two phases (element gradients at the Gauss points, then one thread per
-    node block): within 1e-12 of the oracle on distorted elements with
-    per-element moduli, deterministic, and a re-assembly overwrites earlier
-    values (penalty included); with the block-sliced copy built it is written
-    too (the products stay bitwise those of the ELL values)."""
+    """Two phases (J^-1 and w detJ E of every element at its Gauss points,
+    then the node gather reading them): within 1e-12 of the oracle on
+    distorted elements with per-element moduli, bitwise equal to the node
+    gather that forms the geometry itself where that gather also takes every
+    row of a node in one lane (not Brick20: one row per lane, the sums in
+    another order), deterministic, and a re-assembly
+    overwrites earlier values (penalty included); with the block-sliced copy
+    built it is written too (the products stay bitwise those of the ELL
+    values)."""
     g = gf.Grid(t, nx, ny, nz)
     rng = np.random.default_rng(37)
     E = rng.uniform(1e5, 3e5, g.nelem)
     g_, conn, coords, mesh, K, Ko = build_both(gpu_ctx, t, nx, ny, nz, ng, distort=0.15,
                                                elem_E=E, seed=8)
+    gpu_ctx.assemble_tune(mesh, K, 3)
+    gpu_ctx.assemble(mesh, K)
+    v3, _, _ = gpu_ctx.download(K)
     gpu_ctx.assemble_tune(mesh, K, 4)
     gpu_ctx.assemble(mesh, K)
     v1, _, _ = compare(gpu_ctx, K, Ko)
+    if t != gf.GF_BRICK20:
+        assert np.array_equal(v1, v3)
     gpu_ctx.apply_penalty(K, np.arange(0, K.n, 7, dtype=np.int32))
     gpu_ctx.assemble(mesh, K)
     v2, _, _ = compare(gpu_ctx, K, Ko)
