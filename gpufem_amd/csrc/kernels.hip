@@ -1055,12 +1055,15 @@ hipError_t launch_gather_cfg(hipStream_t s, const GArgs& g) {
 // from the compiler's resource report):
 //  2D: node units (both rows), 2 lanes per node; Q8 1024x512 0.62 ms (one lane
 //      per node: 0.86 ms, 43 KB of LDS per workgroup, 1 wavefront per SIMD).
-//  Brick8: node units, 4 lanes per node, 16 nodes per workgroup, neighbours in
-//      chunks of 4, Gauss-point loop unrolled by 2 (the table loads of two
-//      points batched: 4.38 -> 4.18 ms; by 4: 4.75 ms; the 2D and Brick20
-//      kernels gain nothing or lose): 128^3 4.2 ms; row units (one row per unit,
-//      each element's geometry once per row): 6.3-6.6 ms with 2 lanes per row,
-//      9.7 ms with one lane per row.
+//  Brick8: node units, 8 lanes per node, 8 nodes per workgroup (15.5 KB of
+//      LDS: up to 8 one-wave workgroups per CU, as the registers allow; 4 x 16
+//      with 31 KB fits 5: 4.18 ms), neighbours in chunks of 4 (all 8 at once:
+//      6.0 ms), Gauss-point loop unrolled by 2 (the table loads of two points
+//      batched: 4.38 -> 4.18 ms at 4 x 16; by 4: 4.75 ms; unrolled by 1 at
+//      8 x 8: 4.29 ms; the 2D and Brick20 kernels gain nothing or lose):
+//      128^3 4.06-4.08 ms; row units (one row per unit, each element's
+//      geometry once per row): 6.3-6.6 ms with 2 lanes per row, 9.7 ms with
+//      one lane per row.
 //  Brick20: row units, 4 lanes per row (256 VGPRs + AGPRs): 48^3 8.1-8.7 ms;
 //      node units need chunks of at most 4 neighbours for their registers and
 //      recompute the 20-node geometry per chunk (10.8-20.5 ms).
@@ -1206,8 +1209,8 @@ hipError_t launch_assembly_gather(hipStream_t s, const AssemblyArgs& a) {
     case GF_Q4: return g3 ? launch_gather2<4, 9, 9>(s, g) : launch_gather2<4, 4, 9>(s, g);
     case GF_Q8: return g3 ? launch_gather2<8, 9, 21>(s, g) : launch_gather2<8, 4, 21>(s, g);
     case GF_BRICK8:
-      return g3 ? launch_gather_cfg<3, 8, 27, 27, 3, 4, 16, 4, 2>(s, g)
-                : launch_gather_cfg<3, 8, 8, 27, 3, 4, 16, 4, 2>(s, g);
+      return g3 ? launch_gather_cfg<3, 8, 27, 27, 3, 8, 8, 4, 2>(s, g)
+                : launch_gather_cfg<3, 8, 8, 27, 3, 8, 8, 4, 2>(s, g);
     case GF_BRICK20:
       return g3 ? launch_gather_cfg<3, 20, 27, 81, 1, 4, 16, 20>(s, g)
                 : launch_gather_cfg<3, 20, 8, 81, 1, 4, 16, 20>(s, g);
