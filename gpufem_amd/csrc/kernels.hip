@@ -453,7 +453,11 @@ struct GArgs {
 // unit: lane j of the unit takes the node's incidences j, j + LPU, ... and the
 // LPU lanes add their element's rows into the unit's LDS accumulators in LPU
 // masked phases, so every accumulator receives the elements in ascending
-// incidence order (the sums do not depend on LPU).  UPW units per 64-lane
+// incidence order when the neighbours form one chunk (BCH = NEN).  With
+// smaller chunks the LPU elements of a trip add chunk by chunk, and a slot
+// that is neighbour b of one element and a later-chunk neighbour of an
+// earlier one receives them out of order: the sums then depend on LPU in
+// their last bits (the same configuration gives the same bits).  UPW units per 64-lane
 // workgroup: LPU > 1 divides the LDS per workgroup by LPU and lets more
 // wavefronts share a CU.
 // With RPT == DIM (every row of the node in one unit) the lane forms the
@@ -1190,8 +1194,8 @@ hipError_t launch_assembly_geom(hipStream_t s, const AssemblyArgs& a, double* ge
       return g3 ? launch_geom<2, 8, 9, 21, 2, 30, 8, 1>(s, a, geom)
                 : launch_geom<2, 8, 4, 21, 2, 30, 8, 1>(s, a, geom);
     case GF_BRICK8:
-      return g3 ? launch_geom<3, 8, 27, 27, 4, 16, 4, 2>(s, a, geom)
-                : launch_geom<3, 8, 8, 27, 4, 16, 4, 2>(s, a, geom);
+      return g3 ? launch_geom<3, 8, 27, 27, 8, 8, 4, 2>(s, a, geom)
+                : launch_geom<3, 8, 8, 27, 8, 8, 4, 2>(s, a, geom);
     case GF_BRICK20:
       return g3 ? launch_geom<3, 20, 27, 81, 8, 8, 10, 1>(s, a, geom)
                 : launch_geom<3, 20, 8, 81, 8, 8, 10, 1>(s, a, geom);
