@@ -27,6 +27,7 @@ struct MultiState {
   int32_t col_done[S];  // 1: converged (or empty) column
   int32_t col_status[S];
   int32_t it, done, status, maxit;
+  uint32_t arrived;  // workgroups of k_b_beta finished in this iteration
   double rtol2;
 };
 
@@ -93,6 +94,7 @@ __global__ __launch_bounds__(kNT) void k_b_init_finish(MultiBuffers B, int nb, i
     st->done = all;
     st->status = GF_OK;
     st->maxit = maxit;
+    st->arrived = 0;
     st->rtol2 = rtol2;
   }
 }
@@ -153,20 +155,19 @@ __global__ __launch_bounds__(kNT) void k_b_spmm_pq(MultiBuffers B) {
 }
 
 // alpha_k = rho_k / p_k.q_k (0 for finished columns; a column with
-// p.q <= 0 breaks down and stops)
+// p.q <= 0 breaks down and stops).  One workgroup per column k.
 template <int S>
 __global__ __launch_bounds__(kNT) void k_b_alpha(MultiBuffers B, int nb) {
   MultiState<S>* st = static_cast<MultiState<S>*>(B.st);
   if (st->done) return;
   __shared__ double sh[kNT];
-  __shared__ double pq[S];
-  reduce_columns<S>(B.part1, nb, 0, sh, pq);
-  if (threadIdx.x < S) {
-    const int k = threadIdx.x;
+  const int k = blockIdx.x;
+  const double pq = reduce_column(B.part1, nb, k, sh);
+  if (threadIdx.x == 0) {
     double a = 0.0;
     if (!st->col_done[k]) {
-      if (pq[k] > 0.0) {
-        a = st->rho[k] / pq[k];
+      if (pq > 0.0) {
+        a = st->rho[k] / pq;
       } else {
         st->col_status[k] = GF_ERR_BREAKDOWN;
         st->col_done[k] = 1;
@@ -203,37 +204,43 @@ __global__ __launch_bounds__(kNT) void k_b_update(MultiBuffers B) {
   }
 }
 
-// beta_k, the per-column stopping test and the iteration counters
+// beta_k, the per-column stopping test and the iteration counters.  One
+// workgroup per column k; the last of them to finish (an arrival counter)
+// sets the iteration count and, when every column is done, the done flag.
 template <int S>
 __global__ __launch_bounds__(kNT) void k_b_beta(MultiBuffers B, int nb) {
   MultiState<S>* st = static_cast<MultiState<S>*>(B.st);
   if (st->done) return;
   __shared__ double sh[kNT];
-  __shared__ double rz[S], rr[S];
-  reduce_columns<S>(B.part2, nb, 0, sh, rz);
-  reduce_columns<S>(B.part2, nb, S, sh, rr);
-  const int it = st->it + 1;
-  if (threadIdx.x < S) {
-    const int k = threadIdx.x;
+  const int k = blockIdx.x;
+  const double rz = reduce_column(B.part2, nb, k, sh);
+  const double rr = reduce_column(B.part2, nb, S + k, sh);
+  if (threadIdx.x == 0) {
+    const int it = st->it + 1;
     double beta = 0.0;
     if (!st->col_done[k]) {
-      beta = rz[k] / st->rho[k];
-      st->rho[k] = rz[k];
-      st->rr[k] = rr[k];
+      beta = rz / st->rho[k];
+      st->rho[k] = rz;
+      st->rr[k] = rr;
       st->col_it[k] = it;
-      if (rr[k] <= st->rtol2 * st->bnorm2[k] || it >= st->maxit) {
+      if (rr <= st->rtol2 * st->bnorm2[k] || it >= st->maxit) {
         st->col_done[k] = 1;
         beta = 0.0;
       }
     }
     st->beta[k] = beta;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int all = 1;
-    for (int j = 0; j < S; ++j) all &= st->col_done[j];
-    st->it = it;
-    if (all) st->done = 1;
+    // publish this column's state before arriving; the last arrival reads
+    // every column's
+    __threadfence();
+    if (atomicAdd(&st->arrived, 1u) == S - 1) {
+      __threadfence();
+      int all = 1;
+      for (int j = 0; j < S; ++j)
+        all &= __hip_atomic_load(&st->col_done[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      st->arrived = 0;
+      st->it = it;
+      if (all) st->done = 1;
+    }
   }
 }
 
@@ -384,11 +391,11 @@ hipError_t iteration_s(hipStream_t s, const MultiBuffers& B) {
   hipError_t e;
   const dim3 g(B.nblocks), t(kNT);
   if ((e = launch_spmm_pq<S>(s, B)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_b_alpha<S>, dim3(1), t, 0, s, B, B.nblocks);
+  hipLaunchKernelGGL(k_b_alpha<S>, dim3(S), t, 0, s, B, B.nblocks);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_b_update<S>, g, t, 0, s, B);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_b_beta<S>, dim3(1), t, 0, s, B, B.nblocks);
+  hipLaunchKernelGGL(k_b_beta<S>, dim3(S), t, 0, s, B, B.nblocks);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_b_direction<S>, g, t, 0, s, B);
   return hipGetLastError();

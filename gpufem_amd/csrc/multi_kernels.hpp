@@ -87,6 +87,25 @@ __device__ inline void reduce_columns(const double* part, int nb, int base, doub
   __syncthreads();
 }
 
+// One workgroup per column: the sum over b of part[col * nb + b] in a fixed
+// order (thread t adds b = t, t + kNT, ... in turn, then a tree over the
+// workgroup).  The result is returned to every thread.
+__device__ inline double reduce_column(const double* __restrict__ part, int nb, int col,
+                                       double* sh) {
+  const double* src = part + int64_t(col) * nb;
+  double t = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kNT) t += src[b];
+  __syncthreads();
+  sh[threadIdx.x] = t;
+  __syncthreads();
+#pragma unroll
+  for (int off = kNT / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
 constexpr int kC = 4;
 // load cases per lane in the FP64 block SpMM.  8 would halve the lanes that
 // receive each block's 72 bytes of values, but a workgroup pass then covers
