@@ -34,7 +34,8 @@ struct MMState {
   double rz_new[S];  // r.z after the last update
   int32_t col_it[S], col_done[S], col_status[S], col_refresh[S], col_refinements[S];
   int32_t col_stall[S];  // consecutive refreshes that less than halved the true residual
-  int32_t it, done, status, maxit, any_refresh, pad;
+  int32_t it, done, status, maxit, any_refresh;
+  int32_t arrived;  // workgroups of k_mm_beta finished in this iteration
   double rtol2, delta2;
 };
 
@@ -143,6 +144,7 @@ __global__ __launch_bounds__(kNT) void k_mm_setup_finish(MixedMultiBuffers B, in
     st->status = GF_OK;
     st->maxit = maxit;
     st->any_refresh = 0;
+    st->arrived = 0;
     st->rtol2 = rtol2;
     st->delta2 = delta2;
   }
@@ -181,35 +183,28 @@ __global__ __launch_bounds__(kNT) void k_mm_spmm_pq(MixedMultiBuffers B) {
 
 // alpha_k = rho_k / p_k.q_k (0 for finished columns).  p.q <= 0 in any
 // column (an FP32 operator that is not positive definite) ends the whole
-// solve: the host then redoes every column in FP64.
+// solve: the host then redoes every column in FP64 (so the columns whose
+// workgroups then see the solve done and skip their alpha lose nothing).
+// One workgroup per column k.
 template <int S>
 __global__ __launch_bounds__(kNT) void k_mm_alpha(MixedMultiBuffers B, int nb) {
   MMState<S>* st = static_cast<MMState<S>*>(B.st);
   if (st->done) return;
   __shared__ double sh[kNT];
-  __shared__ double pq[S];
-  __shared__ int broke[S];
-  reduce_columns<S>(B.part1, nb, 0, sh, pq);
-  if (threadIdx.x < S) {
-    const int k = threadIdx.x;
+  const int k = blockIdx.x;
+  const double pq = reduce_column(B.part1, nb, k, sh);
+  if (threadIdx.x == 0) {
     double a = 0.0;
-    broke[k] = 0;
     if (!st->col_done[k]) {
-      if (pq[k] > 0.0) {
-        a = st->rho[k] / pq[k];
+      if (pq > 0.0) {
+        a = st->rho[k] / pq;
       } else {
         st->col_status[k] = GF_ERR_BREAKDOWN;
         st->col_done[k] = 1;
-        broke[k] = 1;
+        st->done = 1;
       }
     }
     st->alpha[k] = a;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int any = 0;
-    for (int j = 0; j < S; ++j) any |= broke[j];
-    if (any) st->done = 1;
   }
 }
 
@@ -242,62 +237,57 @@ __global__ __launch_bounds__(kNT) void k_mm_update(MixedMultiBuffers B) {
 }
 
 // which columns refresh: the recursive residual fell by delta since the
-// column's last refresh, or claims convergence
+// column's last refresh, or claims convergence.  One workgroup per column k;
+// any_refresh (0 since the last beta) is set by every column that is due.
 template <int S>
 __global__ __launch_bounds__(kNT) void k_mm_decide(MixedMultiBuffers B, int nb) {
   MMState<S>* st = static_cast<MMState<S>*>(B.st);
   if (st->done) return;
   __shared__ double sh[kNT];
-  __shared__ double rz[S], rr[S];
-  __shared__ int any[S];
-  reduce_columns<S>(B.part2, nb, 0, sh, rz);
-  reduce_columns<S>(B.part2, nb, S, sh, rr);
-  if (threadIdx.x < S) {
-    const int k = threadIdx.x;
-    st->rz_new[k] = rz[k];
-    st->rr_rec[k] = rr[k];
-    const int due = !st->col_done[k] &&
-                    (rr[k] <= st->delta2 * st->rref2[k] || rr[k] <= st->rtol2 * st->bnorm2[k]);
-    st->col_refresh[k] = due;
-    any[k] = due;
-  }
-  __syncthreads();
+  const int k = blockIdx.x;
+  const double rz = reduce_column(B.part2, nb, k, sh);
+  const double rr = reduce_column(B.part2, nb, S + k, sh);
   if (threadIdx.x == 0) {
-    int a = 0;
-    for (int j = 0; j < S; ++j) a |= any[j];
-    st->any_refresh = a;
+    st->rz_new[k] = rz;
+    st->rr_rec[k] = rr;
+    const int due = !st->col_done[k] &&
+                    (rr <= st->delta2 * st->rref2[k] || rr <= st->rtol2 * st->bnorm2[k]);
+    st->col_refresh[k] = due;
+    if (due) st->any_refresh = 1;
   }
 }
 
 // beta, the per-column stopping test (on refreshed FP64 residuals only) and
-// the iteration counters
+// the iteration counters.  One workgroup per column k; the last of them to
+// finish (an arrival counter) sets the iteration count, clears any_refresh
+// and, when every column is done, sets the done flag.
 template <int S>
 __global__ __launch_bounds__(kNT) void k_mm_beta(MixedMultiBuffers B, int nb) {
   MMState<S>* st = static_cast<MMState<S>*>(B.st);
   if (st->done) return;
   __shared__ double sh[kNT];
-  __shared__ double rz2[S], rr2[S];
+  const int k = blockIdx.x;
   const int refreshed = st->any_refresh;
-  if (refreshed) {  // the same value in every thread of the workgroup
-    reduce_columns<S>(B.part3, nb, 0, sh, rz2);
-    reduce_columns<S>(B.part3, nb, S, sh, rr2);
+  double rz2 = 0.0, rr2 = 0.0;
+  if (refreshed) {  // the same value in every thread of every workgroup
+    rz2 = reduce_column(B.part3, nb, k, sh);
+    rr2 = reduce_column(B.part3, nb, S + k, sh);
   }
-  const int it = st->it + 1;
-  if (threadIdx.x < S) {
-    const int k = threadIdx.x;
+  if (threadIdx.x == 0) {
+    const int it = st->it + 1;
     double beta = 0.0;
     if (!st->col_done[k]) {
       double rz = st->rz_new[k];
       bool stalled = false;
       if (st->col_refresh[k]) {
-        rz = rz2[k];
+        rz = rz2;
         // as the single-RHS solve: kStallRefreshes refreshes in a row that
         // less than halve the true residual stop this column's FP32 iteration
-        const bool met = rr2[k] <= st->rtol2 * st->bnorm2[k];
-        st->col_stall[k] = !met && rr2[k] > 0.25 * st->rref2[k] ? st->col_stall[k] + 1 : 0;
+        const bool met = rr2 <= st->rtol2 * st->bnorm2[k];
+        st->col_stall[k] = !met && rr2 > 0.25 * st->rref2[k] ? st->col_stall[k] + 1 : 0;
         stalled = st->col_stall[k] >= kStallRefreshes;
-        st->rr_true[k] = rr2[k];
-        st->rref2[k] = rr2[k];
+        st->rr_true[k] = rr2;
+        st->rref2[k] = rr2;
         st->col_refinements[k] += 1;
       }
       st->col_it[k] = it;
@@ -320,14 +310,19 @@ __global__ __launch_bounds__(kNT) void k_mm_beta(MixedMultiBuffers B, int nb) {
     }
     st->beta[k] = beta;
     st->col_refresh[k] = 0;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int all = 1;
-    for (int j = 0; j < S; ++j) all &= st->col_done[j];
-    st->it = it;
-    st->any_refresh = 0;
-    if (all) st->done = 1;
+    // publish this column's state before arriving; the last arrival reads
+    // every column's
+    __threadfence();
+    if (atomicAdd(&st->arrived, 1) == S - 1) {
+      __threadfence();
+      int all = 1;
+      for (int j = 0; j < S; ++j)
+        all &= __hip_atomic_load(&st->col_done[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      st->arrived = 0;
+      st->it = it;
+      st->any_refresh = 0;
+      if (all) st->done = 1;
+    }
   }
 }
 
@@ -369,19 +364,19 @@ hipError_t setup_s(hipStream_t s, const MixedMultiBuffers& B, double rtol, doubl
 template <int S>
 hipError_t iteration_s(hipStream_t s, const MixedMultiBuffers& B) {
   hipError_t e;
-  const dim3 g(B.nblocks), t(kNT), one(1);
+  const dim3 g(B.nblocks), t(kNT);
   if (B.bsell_d == 2) hipLaunchKernelGGL((k_mm_spmm_pq<S, 2>), g, t, 0, s, B);
   else if (B.bsell_d == 3) hipLaunchKernelGGL((k_mm_spmm_pq<S, 3>), g, t, 0, s, B);
   else hipLaunchKernelGGL((k_mm_spmm_pq<S, 0>), g, t, 0, s, B);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_mm_alpha<S>, one, t, 0, s, B, B.nblocks);
+  hipLaunchKernelGGL(k_mm_alpha<S>, dim3(S), t, 0, s, B, B.nblocks);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_mm_update<S>, g, t, 0, s, B);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_mm_decide<S>, one, t, 0, s, B, B.nblocks);
+  hipLaunchKernelGGL(k_mm_decide<S>, dim3(S), t, 0, s, B, B.nblocks);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = refresh_s<S>(s, B, 0)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_mm_beta<S>, one, t, 0, s, B, B.nblocks);
+  hipLaunchKernelGGL(k_mm_beta<S>, dim3(S), t, 0, s, B, B.nblocks);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_mm_direction<S>, g, t, 0, s, B);
   return hipGetLastError();
