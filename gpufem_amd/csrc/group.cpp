@@ -263,6 +263,9 @@ int gf_group_create(gf_ctx* ctx, int32_t nmesh, const gf_mesh_desc* meshes, gf_g
   alloc(&g->per_mesh, size_t(7 * nmesh) * 8);
   alloc(&g->per_mesh_i, size_t(3 * nmesh) * 4);
   alloc(&g->st, sizeof(GroupState));
+  // the arrival counter of the per-mesh reductions starts at zero (each of
+  // those kernels leaves it at zero)
+  if (e == hipSuccess) e = hipMemset(g->st, 0, sizeof(GroupState));
   if (e == hipSuccess) e = hipMemcpy(g->d_off, g->off.data(), g->off.size() * 8, hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(g->d_tile_ptr, g->tile_ptr.data(), g->tile_ptr.size() * 8, hipMemcpyHostToDevice);

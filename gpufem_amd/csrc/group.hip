@@ -8,8 +8,8 @@
 // beta, stopping test and iteration count; the tiles of a converged mesh
 // are skipped.  Workgroups take tiles in grid-stride order with the same
 // XCD remap as the single-system kernels.  One iteration is five launches:
-// SpMV + p.q partials, per-mesh alpha (one workgroup), update + partials,
-// per-mesh beta and stopping test (one workgroup), direction.
+// SpMV + p.q partials, per-mesh alpha (a workgroup per mesh), update +
+// partials, per-mesh beta and stopping test (a workgroup per mesh), direction.
 #include <cfloat>
 
 #include "device_common.hpp"
@@ -31,6 +31,14 @@ __device__ inline void tiles_loop(const GroupBuffers& G, F&& f) {
 }
 
 __device__ inline double tile_sum(double v, double* sh) { return block_sum(v, sh); }
+
+// v[a] + ... + v[b - 1] over one workgroup in a fixed order (thread t adds
+// a + t, a + t + kNT, ... in turn, then the workgroup sum); in every thread
+__device__ inline double run_sum(const double* __restrict__ v, int64_t a, int64_t b, double* sh) {
+  double t = 0.0;
+  for (int64_t s = a + threadIdx.x; s < b; s += kNT) t += v[s];
+  return block_sum(t, sh);
+}
 
 // ---- penalty: pen[m] = coef * max diagonal of mesh m; K_dd *= pen[m] -------
 __global__ __launch_bounds__(kNT) void k_g_diag_max(GroupBuffers G, double coef) {
@@ -92,15 +100,17 @@ __global__ __launch_bounds__(kNT) void k_g_init(GroupBuffers G, int use_x0) {
   });
 }
 
+// one workgroup per mesh m; the last to finish (an arrival counter) sets the
+// state of the whole group
 __global__ __launch_bounds__(kNT) void k_g_init_finish(GroupBuffers G, double rtol2, int maxit) {
   GroupState* st = G.st;
-  for (int m = threadIdx.x; m < G.nmesh; m += kNT) {
-    double rz = 0, rr = 0, bb = 0;
-    for (int64_t t = G.tile_ptr[m]; t < G.tile_ptr[m + 1]; ++t) {
-      rz += G.part[t];
-      rr += G.part[G.ntiles + t];
-      bb += G.part[2 * G.ntiles + t];
-    }
+  __shared__ double sh[kNT / 64];
+  const int m = blockIdx.x;
+  const int64_t t0 = G.tile_ptr[m], t1 = G.tile_ptr[m + 1];
+  const double rz = run_sum(G.part, t0, t1, sh);
+  const double rr = run_sum(G.part + G.ntiles, t0, t1, sh);
+  const double bb = run_sum(G.part + 2 * G.ntiles, t0, t1, sh);
+  if (threadIdx.x == 0) {
     G.rho[m] = rz;
     G.rr[m] = rr;
     G.bb[m] = bb;
@@ -109,15 +119,18 @@ __global__ __launch_bounds__(kNT) void k_g_init_finish(GroupBuffers G, double rt
     G.it[m] = 0;
     G.status[m] = GF_OK;
     G.done[m] = (rr <= rtol2 * bb) || maxit <= 0;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int all = 1;
-    for (int m = 0; m < G.nmesh; ++m) all &= G.done[m];
-    st->it = 0;
-    st->done = all;
-    st->maxit = maxit;
-    st->rtol2 = rtol2;
+    __threadfence();
+    if (atomicAdd(&st->arrived, 1) == G.nmesh - 1) {
+      __threadfence();
+      int all = 1;
+      for (int j = 0; j < G.nmesh; ++j)
+        all &= __hip_atomic_load(&G.done[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      st->arrived = 0;
+      st->it = 0;
+      st->done = all;
+      st->maxit = maxit;
+      st->rtol2 = rtol2;
+    }
   }
 }
 
@@ -186,22 +199,24 @@ __global__ __launch_bounds__(kNT) void k_g_spmv_pq_bs(GroupBuffers G) {
   }
 }
 
+// one workgroup per mesh m
 __global__ __launch_bounds__(kNT) void k_g_alpha(GroupBuffers G) {
   if (G.st->done) return;
-  for (int m = threadIdx.x; m < G.nmesh; m += kNT) {
+  __shared__ double sh[kNT / 64];
+  const int m = blockIdx.x;
+  if (G.done[m]) {
+    if (threadIdx.x == 0) G.alpha[m] = 0.0;
+    return;
+  }
+  const double pq = G.bsell_d ? run_sum(G.part_sg, G.sg_ptr[m], G.sg_ptr[m + 1], sh)
+                              : run_sum(G.part, G.tile_ptr[m], G.tile_ptr[m + 1], sh);
+  if (threadIdx.x == 0) {
     double a = 0.0;
-    if (!G.done[m]) {
-      double pq = 0.0;
-      if (G.bsell_d)
-        for (int64_t s = G.sg_ptr[m]; s < G.sg_ptr[m + 1]; ++s) pq += G.part_sg[s];
-      else
-        for (int64_t t = G.tile_ptr[m]; t < G.tile_ptr[m + 1]; ++t) pq += G.part[t];
-      if (pq > 0.0) {
-        a = G.rho[m] / pq;
-      } else {
-        G.status[m] = GF_ERR_BREAKDOWN;
-        G.done[m] = 1;
-      }
+    if (pq > 0.0) {
+      a = G.rho[m] / pq;
+    } else {
+      G.status[m] = GF_ERR_BREAKDOWN;
+      G.done[m] = 1;
     }
     G.alpha[m] = a;
   }
@@ -239,37 +254,44 @@ __global__ __launch_bounds__(kNT) void k_g_update(GroupBuffers G) {
   });
 }
 
+// one workgroup per mesh m; the last to finish (an arrival counter) sets
+// the iteration count and, when every mesh is done, the done flag
 __global__ __launch_bounds__(kNT) void k_g_beta(GroupBuffers G) {
   GroupState* st = G.st;
   if (st->done) return;
+  __shared__ double sh[kNT / 64];
+  const int m = blockIdx.x;
   const int it = st->it + 1;
-  const double rtol2 = st->rtol2;
-  const int maxit = st->maxit;
-  for (int m = threadIdx.x; m < G.nmesh; m += kNT) {
+  double rz = 0.0, rr = 0.0;
+  const int live = !G.done[m];
+  if (live) {  // the same in every thread of the workgroup
+    const int64_t t0 = G.tile_ptr[m], t1 = G.tile_ptr[m + 1];
+    rz = run_sum(G.part, t0, t1, sh);
+    rr = run_sum(G.part + G.ntiles, t0, t1, sh);
+  }
+  if (threadIdx.x == 0) {
     double beta = 0.0;
-    if (!G.done[m]) {
-      double rz = 0, rr = 0;
-      for (int64_t t = G.tile_ptr[m]; t < G.tile_ptr[m + 1]; ++t) {
-        rz += G.part[t];
-        rr += G.part[G.ntiles + t];
-      }
+    if (live) {
       beta = rz / G.rho[m];
       G.rho[m] = rz;
       G.rr[m] = rr;
       G.it[m] = it;
-      if (rr <= rtol2 * G.bb[m] || it >= maxit) {
+      if (rr <= st->rtol2 * G.bb[m] || it >= st->maxit) {
         G.done[m] = 1;
         beta = 0.0;
       }
     }
     G.beta[m] = beta;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int all = 1;
-    for (int m = 0; m < G.nmesh; ++m) all &= G.done[m];
-    st->it = it;
-    if (all) st->done = 1;
+    __threadfence();
+    if (atomicAdd(&st->arrived, 1) == G.nmesh - 1) {
+      __threadfence();
+      int all = 1;
+      for (int j = 0; j < G.nmesh; ++j)
+        all &= __hip_atomic_load(&G.done[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      st->arrived = 0;
+      st->it = it;
+      if (all) st->done = 1;
+    }
   }
 }
 
@@ -350,7 +372,7 @@ hipError_t launch_group_setup(hipStream_t s, const GroupBuffers& G, int use_x0, 
   }
   hipLaunchKernelGGL(k_g_init, dim3(G.nblocks), dim3(kNT), 0, s, G, use_x0);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_g_init_finish, dim3(1), dim3(kNT), 0, s, G, rtol * rtol, maxit);
+  hipLaunchKernelGGL(k_g_init_finish, dim3(G.nmesh), dim3(kNT), 0, s, G, rtol * rtol, maxit);
   return hipGetLastError();
 }
 
@@ -368,11 +390,11 @@ hipError_t launch_group_iteration(hipStream_t s, const GroupBuffers& G) {
     default: hipLaunchKernelGGL(k_g_spmv_pq<kThreadU8>, g, t, 0, s, G); break;
   }
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_g_alpha, dim3(1), t, 0, s, G);
+  hipLaunchKernelGGL(k_g_alpha, dim3(G.nmesh), t, 0, s, G);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_g_update, g, t, 0, s, G);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_g_beta, dim3(1), t, 0, s, G);
+  hipLaunchKernelGGL(k_g_beta, dim3(G.nmesh), t, 0, s, G);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_g_direction, g, t, 0, s, G);
   return hipGetLastError();

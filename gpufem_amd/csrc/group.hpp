@@ -8,7 +8,8 @@
 namespace gf {
 
 struct GroupState {
-  int32_t it, done, maxit, pad;
+  int32_t it, done, maxit;
+  int32_t arrived;  // workgroups of k_g_beta finished in this iteration
   double rtol2;
 };
 
