@@ -24,7 +24,7 @@ struct gf_group {
   // device arrays
   int64_t *d_off = nullptr, *d_tile_ptr = nullptr, *d_tile_r0 = nullptr, *d_tile_r1 = nullptr;
   int32_t* d_tile_mesh = nullptr;
-  double *b = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  double *b = nullptr, *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;
   double* part = nullptr;
   // slice groups of the block-sliced SpMV (built on the first solve that uses it)
   int64_t nsg = 0;
@@ -62,7 +62,7 @@ void group_release(gf_group* g) {
   dfree(g->d_tile_r0);
   dfree(g->d_tile_r1);
   dfree(g->d_tile_mesh);
-  double** v[] = {&g->b, &g->x, &g->r, &g->z, &g->p, &g->q, &g->part, &g->per_mesh};
+  double** v[] = {&g->b, &g->x, &g->r, &g->p, &g->q, &g->part, &g->per_mesh};
   for (double** d : v) dfree(*d);
   dfree(g->per_mesh_i);
   dfree(g->d_sg_ptr);
@@ -154,7 +154,6 @@ GroupBuffers buffers(gf_group* g) {
   G.b = g->b;
   G.x = g->x;
   G.r = g->r;
-  G.z = g->z;
   G.p = g->p;
   G.q = g->q;
   G.part = g->part;
@@ -258,7 +257,7 @@ int gf_group_create(gf_ctx* ctx, int32_t nmesh, const gf_mesh_desc* meshes, gf_g
   alloc(&g->d_tile_r0, tr0.size() * 8);
   alloc(&g->d_tile_r1, tr1.size() * 8);
   alloc(&g->d_tile_mesh, tmesh.size() * 4);
-  for (double** d : {&g->b, &g->x, &g->r, &g->z, &g->p, &g->q}) alloc(d, vb);
+  for (double** d : {&g->b, &g->x, &g->r, &g->p, &g->q}) alloc(d, vb);
   alloc(&g->part, size_t(3 * g->ntiles) * 8);
   alloc(&g->per_mesh, size_t(7 * nmesh) * 8);
   alloc(&g->per_mesh_i, size_t(3 * nmesh) * 4);
@@ -410,6 +409,7 @@ int gf_group_pcg(gf_ctx* ctx, gf_group* g, const double* b, double* x, const gf_
         for (int it = 0; it < kChunk; ++it) GF_HIP_CHECK(ctx, launch_group_iteration(ctx->stream, G));
       }
     }
+    GF_HIP_CHECK(ctx, launch_group_x_finish(ctx->stream, G));
     GF_HIP_CHECK(ctx, launch_group_residual(ctx->stream, G, d_res));
     GF_HIP_CHECK(ctx, hipMemcpyAsync(pm.data(), g->per_mesh, pm.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
     GF_HIP_CHECK(ctx, hipMemcpyAsync(pi.data(), g->per_mesh_i, pi.size() * 4, hipMemcpyDeviceToHost, ctx->stream));

@@ -82,7 +82,6 @@ __global__ __launch_bounds__(kNT) void k_g_init(GroupBuffers G, int use_x0) {
       if (!use_x0) G.x[i] = 0.0;
       const double zi = G.dinv[i] * ri;
       G.r[i] = ri;
-      G.z[i] = zi;
       G.p[i] = zi;
       rz = ri * zi;
       rr = ri * ri;
@@ -236,12 +235,10 @@ __global__ __launch_bounds__(kNT) void k_g_update(GroupBuffers G) {
     const int64_t i = G.tile_r0[t] + threadIdx.x;
     const double a = G.alpha[m];
     double rz = 0, rr = 0;
-    if (i < G.tile_r1[t]) {
-      G.x[i] += a * G.p[i];
+    if (i < G.tile_r1[t]) {  // x is updated with p in k_g_direction
       const double ri = G.r[i] - a * G.q[i];
       const double zi = G.dinv[i] * ri;
       G.r[i] = ri;
-      G.z[i] = zi;
       rz = ri * zi;
       rr = ri * ri;
     }
@@ -295,11 +292,27 @@ __global__ __launch_bounds__(kNT) void k_g_beta(GroupBuffers G) {
   }
 }
 
+// x += alpha p with this iteration's p (alpha = 0 for meshes done before
+// it), then p = D^-1 r + beta p.  The iteration that finishes the group
+// returns here before the x update (done is set); k_g_x_finish applies it.
 __global__ __launch_bounds__(kNT) void k_g_direction(GroupBuffers G) {
   if (G.st->done) return;
   tiles_loop(G, [&](int64_t t, int m) {
     const int64_t i = G.tile_r0[t] + threadIdx.x;
-    if (i < G.tile_r1[t]) G.p[i] = G.z[i] + G.beta[m] * G.p[i];
+    if (i < G.tile_r1[t]) {
+      const double pk = G.p[i];
+      G.x[i] += G.alpha[m] * pk;
+      G.p[i] = G.dinv[i] * G.r[i] + G.beta[m] * pk;
+    }
+  });
+}
+
+// the x update of the group's last iteration (alpha = 0 for meshes finished
+// before it, and after the setup)
+__global__ __launch_bounds__(kNT) void k_g_x_finish(GroupBuffers G) {
+  tiles_loop(G, [&](int64_t t, int m) {
+    const int64_t i = G.tile_r0[t] + threadIdx.x;
+    if (i < G.tile_r1[t]) G.x[i] += G.alpha[m] * G.p[i];
   });
 }
 
@@ -397,6 +410,11 @@ hipError_t launch_group_iteration(hipStream_t s, const GroupBuffers& G) {
   hipLaunchKernelGGL(k_g_beta, dim3(G.nmesh), t, 0, s, G);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_g_direction, g, t, 0, s, G);
+  return hipGetLastError();
+}
+
+hipError_t launch_group_x_finish(hipStream_t s, const GroupBuffers& G) {
+  hipLaunchKernelGGL(k_g_x_finish, dim3(G.nblocks), dim3(kNT), 0, s, G);
   return hipGetLastError();
 }
 

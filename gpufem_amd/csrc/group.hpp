@@ -31,7 +31,7 @@ struct GroupBuffers {
   const int32_t* diag;
   const double* dinv;
   const double* b;
-  double *x, *r, *z, *p, *q;
+  double *x, *r, *p, *q;  // z = D^-1 r is formed where it is used
   double* part;  // [3 * ntiles]
   // block-sliced SpMV (bsell_d = 2, 3; 0: thread per row over col/scol):
   // slice groups of at most four slices of one mesh, one workgroup each
@@ -58,6 +58,8 @@ hipError_t launch_group_penalty(hipStream_t s, const GroupBuffers& G, double coe
 hipError_t launch_group_setup(hipStream_t s, const GroupBuffers& G, int use_x0, double rtol,
                               int maxit);
 hipError_t launch_group_iteration(hipStream_t s, const GroupBuffers& G);
+// x += alpha p of the group's last iteration (once, after the loop)
+hipError_t launch_group_x_finish(hipStream_t s, const GroupBuffers& G);
 hipError_t launch_group_residual(hipStream_t s, const GroupBuffers& G, double* out);
 
 }  // namespace gf
