@@ -6,8 +6,8 @@ traction, x = 0 clamped) the grouped FP64 Jacobi-PCG computes displacement
 fields; the network of libgpufem (gf_nn_*, a BF16 3D U-Net on the node grid
 with hand-written HIP forward, backward and Adam kernels) learns
 features -> displacement; its prediction becomes the PCG initial guess.
-Zero-start PCG is the fallback and the judge: every warm-started solve must
-reach the same true residual ||b - K u|| <= rtol ||b||.
+Zero-start PCG is the fallback and the yardstick: every warm-started solve
+must reach the same true residual ||b - K u|| <= rtol ||b||.
 
 Features per node (16 channels, bf16):
   0      log(E / 210000) / sigma, averaged over the node's elements
@@ -387,8 +387,8 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--group", type=int, default=16)
     ap.add_argument("--lr", type=float, default=5e-4)
-    ap.add_argument("--out", default="gpurun_out/learned_brick8_cantilever.npz")
-    ap.add_argument("--report", default="gpurun_out/learned_report.json")
+    ap.add_argument("--out", default="learned_brick8_cantilever.npz")
+    ap.add_argument("--report", default="learned_report.json")
     args = ap.parse_args(argv)
     ctx = gf.Context(0)
     fam = Family()

@@ -1,8 +1,8 @@
 """ctypes binding of the FP64 CPU oracle (oracle/liboracle.so).
 
-TEST INFRASTRUCTURE ONLY: the checker for the HIP product path.  Only tests/,
-__graft_entry__.smoke() and bench.py's cpu_baseline leg may import this
-package; the product (gpufem_amd) never does.
+TEST INFRASTRUCTURE ONLY: the oracle the HIP product path is tested against.
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this package; the product (gpufem_amd) never does.
 """
 import ctypes as C
 import os

@@ -1,7 +1,7 @@
 """CPU restatement of the learned warm-start network (numpy).
 
-TEST INFRASTRUCTURE ONLY: the checker for the HIP network kernels of
-gpufem_amd (csrc/nn.hip, csrc/nn.cpp).  Only tests/ may import it.
+TEST INFRASTRUCTURE ONLY: the oracle the HIP network kernels of gpufem_amd
+(csrc/nn.hip, csrc/nn.cpp) are tested against.  Only tests/ may import it.
 
 The reference has no learned component; this module restates textbook
 definitions (3x3x3 zero-padded convolution, strided and transposed; trilinear

@@ -3,9 +3,9 @@
  * path (Gauss-quadrature Ke for Q4/Q8/Brick8/Brick20, colored scatter into
  * slot-major ELL, penalty constraints, CG and Jacobi-PCG).
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP product path.  Only
- * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
- * it; the product library never links or calls it.
+ * TEST INFRASTRUCTURE ONLY: the oracle the HIP product path is tested
+ * against.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it; the product library never links or calls it.
  *
  * What pins each function (checks that do not restate the formula):
  *   orc_gauss_points      closed forms: sum w = 2^dim; integrates x^k exactly

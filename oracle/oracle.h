@@ -1,9 +1,10 @@
 /*
  * oracle.h -- FP64 CPU restatement of fpaboim/gpufem's assembly-and-solve path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
- * path: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
- * may load it.  It is never linked into, or called by, the product library.
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle the HIP product path
+ * is tested against: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  It is never linked into, or called by, the
+ * product library.
  *
  * Every function cites the reference file:line it restates (paths relative to
  * the reference repository root).  Arithmetic is IEEE double throughout; the

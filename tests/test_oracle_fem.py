@@ -181,7 +181,7 @@ def test_grid_positive_jacobian_and_counts(t, nx, ny, nz, ng):
         assert np.allclose(xe, lo + (xi + 1) / 2, atol=1e-14)
 
 
-def test_survey_sizes():
+def test_benchmark_grid_sizes():
     expect = {(gf.GF_Q4, 64, 32, 0): (2145, 2048), (gf.GF_Q8, 1024, 512, 0): (1575937, 524288),
               (gf.GF_BRICK8, 128, 128, 128): (2146689, 2097152),
               (gf.GF_BRICK20, 48, 48, 48): (463393, 110592),
@@ -206,7 +206,7 @@ def test_face_loads_sum_to_total_force(t):
     assert set(loaded.tolist()) <= set(face.tolist())
 
 
-def test_q4_plate_config_matches_survey():
+def test_q4_plate_config_sizes():
     p = P.config_q4_plate()
     assert p.ndof == 4290 and p.conn.shape == (2048, 4)
     assert p.fixed_dofs.size == 66
