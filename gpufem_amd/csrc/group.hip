@@ -73,6 +73,8 @@ __global__ void k_g_penalty_apply(GroupBuffers G, const int64_t* dofs, const int
 // ---- segmented PCG ------------------------------------------------------------
 __global__ __launch_bounds__(kNT) void k_g_init(GroupBuffers G, int use_x0) {
   __shared__ double sh[kNT / 64];
+  // the arrival counter of k_g_init_finish, which follows
+  if (blockIdx.x == 0 && threadIdx.x == 0) G.st->arrived = 0;
   tiles_loop(G, [&](int64_t t, int m) {
     const int64_t i = G.tile_r0[t] + threadIdx.x;
     double rz = 0, rr = 0, bb = 0;
