@@ -265,6 +265,7 @@ template <int S, int D, class T, class VT, bool DIFF = false, int NB = 2, bool S
           int KC = kC, bool SHARE = false, class F>
 __device__ inline void for_chunk_node_rows(const SpmmArgs& A, const VT* __restrict__ val,
                                            const T* __restrict__ xin, F&& f) {
+  static_assert(!(DIFF && SPLIT), "the difference form reads x_R in the contiguous order");
   int g, c0;
   chunk_lane<S, KC>(&g, &c0);
   const int64_t npos = A.bs.nslices * 64;
