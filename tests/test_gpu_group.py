@@ -52,6 +52,20 @@ def test_group_matches_single_mesh_solver(gpu_ctx):
         assert np.linalg.norm(xk - xs) <= 1e-9 * np.linalg.norm(xs)
 
 
+def test_group_solve_is_deterministic(gpu_ctx):
+    """The per-mesh reductions (one workgroup per mesh, the last to finish
+    setting the group's state) add in a fixed order: the same solve twice
+    gives the same bits and per-mesh iteration counts."""
+    probs = [P.structured(gf.GF_Q4, nx, ny) for nx, ny in [(16, 8), (5, 3), (24, 12)]]
+    G = gpu_ctx.group(probs)
+    G.assemble()
+    G.apply_penalty()
+    x1, s1, its1 = G.pcg()
+    x2, s2, its2 = G.pcg()
+    assert s1.converged
+    assert np.array_equal(x1, x2) and np.array_equal(its1, its2)
+
+
 def test_group_rejects_mixed_element_types(gpu_ctx):
     with pytest.raises(gf.GpufemError):
         gpu_ctx.group([P.structured(gf.GF_Q4, 4, 2), P.structured(gf.GF_Q8, 4, 2)])

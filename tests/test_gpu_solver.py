@@ -197,6 +197,25 @@ def test_batched_pcg_matches_column_solves(gpu_ctx, nrhs, elem, variant):
         assert np.linalg.norm(Xt[k] - xo) <= 1e-9 * max(np.linalg.norm(xo), 1e-300)
 
 
+@pytest.mark.parametrize("precision", [gf.GF_PREC_FP64, gf.GF_PREC_MIXED])
+@pytest.mark.parametrize("nrhs", [32, 40])
+def test_batched_solves_are_deterministic(gpu_ctx, nrhs, precision):
+    """The per-column reductions (one workgroup per load case, the last to
+    finish setting the iteration state) add in a fixed order: the same solve
+    twice gives the same bits and iteration counts."""
+    p = P.structured(gf.GF_BRICK20, 3, 3, 2)
+    mesh = gpu_ctx.upload_mesh(p.elem_type, p.ngauss, p.conn, p.coords, p.E, p.nu)
+    K = gpu_ctx.symbolic(mesh)
+    gpu_ctx.assemble(mesh, K)
+    gpu_ctx.apply_penalty(K, p.fixed_dofs)
+    gpu_ctx.tune(K, 8)
+    F = P.random_top_loads(p, nrhs, 5)
+    X1, s1 = gpu_ctx.pcg(K, F, precision=precision)
+    X2, s2 = gpu_ctx.pcg(K, F, precision=precision)
+    assert s1.converged and s1.true_relres <= 1.05e-8
+    assert np.array_equal(X1, X2) and s1.iterations == s2.iterations
+
+
 @pytest.mark.parametrize("elem,variant", [(gf.GF_BRICK20, 0), (gf.GF_BRICK20, 8),
                                           (gf.GF_Q8, 0), (gf.GF_Q8, 7)])
 @pytest.mark.parametrize("nrhs", [2, 5, 32, 40])
