@@ -196,7 +196,9 @@ __device__ inline void load_node(const T* __restrict__ x, int64_t c, T* xv) {
 // Entries are added in ascending column order, as row_dot adds its slots.
 // U blocks per step (measured on the Q8 plate and the Brick8 cube: 4 for
 // 2 x 2 blocks, 3 for 3 x 3; fewer lose more in-flight loads than the extra
-// occupancy gives back, more cost occupancy).
+// occupancy gives back, more cost occupancy; the FP32 copy too, whose blocks
+// carry half the bytes: 4 or 6 per step made the cube's mixed solve 1-4 %
+// slower).
 //
 // DIFF: the copy is in difference form (the FP32 copy of the mixed-precision
 // solve): an off-diagonal block K_RJ multiplies x_J - x_R and the diagonal
